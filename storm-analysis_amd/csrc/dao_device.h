@@ -1,0 +1,130 @@
+// Device-side data layout shared by the HIP kernels and the host engine.
+//
+// One "batch" holds F frames.  Pixel arrays are structure-of-arrays over all
+// frames (frame f occupies [pix0, pix0 + W*H)), row-major with x the fast
+// axis, pixel index k = (yi + ry) * W + (xi + rx) (multi_fit.c:50).  Peaks are
+// structure-of-arrays; frame f owns slots [peak0, peak0 + cap) and its live
+// peaks are the first nfit of them, in the reference's index order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dao {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kMaxPred = 128;  // stored predecessors per op (more -> cell rescan)
+constexpr int kNFit = 7;
+
+// parameter slots (multi_fit.h:20-26)
+constexpr int P_HEIGHT = 0, P_XC = 1, P_XW = 2, P_YC = 3, P_YW = 4, P_BG = 5, P_ZC = 6;
+// status (multi_fit.h:33-35)
+constexpr int ST_RUNNING = 0, ST_CONVERGED = 1, ST_ERROR = 2;
+// models
+enum Shape { SH_2DFIXED = 0, SH_2D = 1, SH_3D = 2, SH_Z = 3 };
+enum ErrModel { EM_MLE = 0, EM_ALS = 1, EM_LS = 2, EM_DWLS = 3, EM_FWLS = 4 };
+// counters, in the order of the fitData prefix (multi_fit.h:87-94)
+enum Counter {
+  C_DPOSV = 0, C_ITER, C_LOST, C_MARGIN, C_NEG_FI, C_NEG_HEIGHT, C_NON_CONV, C_NON_DECR, C_COUNT
+};
+// kinds of image operation run through the ordered executor
+enum OpKind { OP_LM = 0, OP_RECENTER = 1, OP_NEWPEAK = 2, OP_SUBTRACT = 3, OP_PEAKERROR = 4 };
+// selection predicates over peak slots (k_flag)
+enum SelectMode { SEL_RUNNING = 0, SEL_RECENTER = 1, SEL_RUNNING_BAD = 2 };
+// start-peak formats of daoNewPeaks (dao_fit.c:1605, 1695)
+enum PeakFmt { PF_FINDER = 0, PF_TESTING = 1, PF_TEXT = 2 };
+
+struct Model {
+  int shape;
+  int errm;
+  int roi;       // fit_size_x == fit_size_y
+  int nroi;      // roi_n_index
+  int jac;       // jac_size
+  int margin;    // reserved-box margin around the ROI, pixels
+  double xoff;   // == yoff
+  double zoff;
+  double tol;
+  double min_height;
+  double wmin, wmax;
+  double zmin, zmax;
+  double wxz[5], wyz[5];  // wxz[0], wyz[0] hold wo^2 (dao_fit.c:1566-1567)
+};
+
+struct Frame {
+  int W, H;
+  long long pix0;
+  int peak0;
+  int cap;
+  int nfit;
+  int cell0;  // first cell of this frame's scheduling grid
+  int ncx, ncy;
+};
+
+struct Pix {
+  double* x;      // image * rqe + scmos
+  double* rqe;
+  double* scmos;  // variance / gain^2
+  double* f;      // fit foreground (rqe applied)
+  double* bg;     // sum of covering peaks' bg*rqe + scmos
+  double* tfi;    // cached transformed fit value per pixel
+  double* bgest;  // background estimate * rqe
+  double* asx;    // Anscombe transform of x (ALS only)
+  int* cnt;       // number of peaks covering the pixel
+  uint8_t* stale; // fit changed since the error was last evaluated here
+};
+
+struct Peaks {
+  double* prm;    // [kNFit][cap_total] param-major
+  double* lam;
+  double* err;
+  int* xi;
+  int* yi;
+  int* status;
+  int* iter;
+  int* added;
+  int* frame;
+  long long cap;  // stride of prm
+  // copy of the state at the start of a sweep attempt (rollback)
+  double* bk_prm;
+  double* bk_lam;
+  double* bk_err;
+  int* bk_xi;
+  int* bk_yi;
+  int* bk_status;
+  int* bk_iter;
+  int* bk_added;
+};
+
+// The ordered executor: ops i < j of one frame whose boxes overlap run in
+// index order; the others run concurrently.
+struct Sched {
+  int nops;
+  int* op_peak;     // peak slot of op i
+  int4* box;        // [x0, y0, x1, y1) per op
+  int* pred_n;      // number of predecessors (> kMaxPred: rescan cells)
+  int* pred;        // [nops][kMaxPred]
+  int* done;        // round in which the op finished, -1 pending
+  int* cell_start;  // per cell, into cell_ops (size ncells + 1)
+  int* cell_ops;    // op ids sorted by cell
+  int* cell_fill;
+  int* big_ops;     // ops whose box is wider than the cell
+  int* n_big;
+  int cell;         // cell edge, pixels
+  int* op_delta;    // [nops][C_COUNT] counter deltas of the op
+  int* op_flags;    // bit0 completed, bit1 escaped
+  int4* esc_box;    // box needed by an escaped op
+  int* frame_bad;   // per frame: an op escaped its box in this attempt
+  int* remaining;   // ops not yet done
+};
+
+struct OpArgs {
+  Model m;
+  Pix px;
+  Peaks pk;
+  Sched sc;
+  const Frame* frames;
+  const int* roi_tab;
+  int round;
+};
+
+}  // namespace dao

@@ -1,0 +1,627 @@
+// Host runtime of the MI355X fitter (see dao_engine.h).
+#include "dao_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace dao {
+
+void fatal(const char* file, int line, const char* what, const char* msg) {
+  std::fprintf(stderr, "dao_fit (HIP): %s:%d: %s failed: %s\n", file, line, what, msg);
+  std::fflush(stderr);
+  std::abort();
+}
+
+static void fail(const char* msg) {
+  std::fprintf(stderr, "dao_fit (HIP): %s\n", msg);
+  std::fflush(stderr);
+  std::abort();
+}
+
+void roi_table(int roi, std::vector<int>& rx, std::vector<int>& ry) {
+  rx.clear();
+  ry.clear();
+  double c = 0.5 * (double)roi;
+  for (int i = 0; i < roi; i++) {
+    double dy = (double)i - c + 0.5;
+    for (int j = 0; j < roi; j++) {
+      double dx = (double)j - c + 0.5;
+      if ((dx * dx + dy * dy) <= c * c) {
+        rx.push_back(j);
+        ry.push_back(i);
+      }
+    }
+  }
+}
+
+template <typename T>
+static T* dalloc(long long n) {
+  void* p = nullptr;
+  if (n <= 0) n = 1;
+  DAO_CHECK(hipMalloc(&p, sizeof(T) * (size_t)n));
+  return (T*)p;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) DAO_CHECK(hipFree((void*)p));
+  p = nullptr;
+}
+
+Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) fail("no HIP device available");
+  DAO_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+
+  roi_table(m.roi, roi_x, roi_y);
+  m.nroi = (int)roi_x.size();
+  if (m.roi < 1 || m.roi > 32) fail("roi_size must be in [1, 32]");
+  std::vector<int> packed(m.nroi);
+  for (int j = 0; j < m.nroi; j++) packed[j] = roi_x[j] | (roi_y[j] << 16);
+  d_roi = dalloc<int>(m.nroi);
+  DAO_CHECK(hipMemcpy(d_roi, packed.data(), sizeof(int) * m.nroi, hipMemcpyHostToDevice));
+
+  F = (int)specs.size();
+  fr.resize(F);
+  int cell = m.roi + 2 * m.margin;
+  long long pix = 0, slots = 0;
+  int cells = 0;
+  for (int f = 0; f < F; f++) {
+    Frame& q = fr[f];
+    q.W = specs[f].W;
+    q.H = specs[f].H;
+    q.pix0 = pix;
+    q.peak0 = (int)slots;
+    q.cap = specs[f].cap;
+    q.nfit = 0;
+    q.cell0 = cells;
+    q.ncx = (q.W + cell - 1) / cell;
+    q.ncy = (q.H + cell - 1) / cell;
+    pix += (long long)q.W * q.H;
+    slots += q.cap;
+    cells += q.ncx * q.ncy;
+  }
+  npix = pix;
+  ncells = cells;
+  d_fr = dalloc<Frame>(F);
+  upload_frames();
+
+  px.x = dalloc<double>(npix);
+  px.rqe = dalloc<double>(npix);
+  px.scmos = dalloc<double>(npix);
+  px.f = dalloc<double>(npix);
+  px.bg = dalloc<double>(npix);
+  px.tfi = dalloc<double>(npix);
+  px.bgest = dalloc<double>(npix);
+  px.asx = dalloc<double>(npix);
+  px.cnt = dalloc<int>(npix);
+  px.stale = dalloc<uint8_t>(npix);
+  DAO_CHECK(hipMemsetAsync(px.f, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.bg, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.tfi, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.bgest, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.asx, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.x, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.cnt, 0, sizeof(int) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.stale, 1, npix, st));
+
+  // scheduling grid (cell_start has one extra entry)
+  sc.cell = cell;
+  sc.cell_start = dalloc<int>(ncells + 1);
+  sc.cell_fill = dalloc<int>(ncells);
+  DAO_CHECK(hipMemsetAsync(sc.cell_fill, 0, sizeof(int) * ncells, st));
+  sc.n_big = dalloc<int>(1);
+  sc.remaining = dalloc<int>(1);
+  sc.frame_bad = dalloc<int>(F);
+  DAO_CHECK(hipMemsetAsync(sc.frame_bad, 0, sizeof(int) * F, st));
+  d_counters = dalloc<int>((long long)F * C_COUNT);
+  DAO_CHECK(hipMemsetAsync(d_counters, 0, sizeof(int) * F * C_COUNT, st));
+  d_count = dalloc<int>(1);
+
+  cap = 0;
+  grow_total_cap(std::max<long long>(slots, 1));
+  DAO_CHECK(hipStreamSynchronize(st));
+}
+
+Engine::~Engine() {
+  if (st) hipStreamSynchronize(st);
+  free_peaks(pk);
+  free_peaks(pk2);
+  free_sched();
+  dfree(d_fr);
+  dfree(px.x);
+  dfree(px.rqe);
+  dfree(px.scmos);
+  dfree(px.f);
+  dfree(px.bg);
+  dfree(px.tfi);
+  dfree(px.bgest);
+  dfree(px.asx);
+  dfree(px.cnt);
+  dfree(px.stale);
+  dfree(sc.cell_start);
+  dfree(sc.cell_fill);
+  dfree(sc.n_big);
+  dfree(sc.remaining);
+  dfree(sc.frame_bad);
+  dfree(d_counters);
+  dfree(d_count);
+  dfree(d_roi);
+  dfree(d_stage);
+  dfree(d_istage);
+  if (h_pin) hipHostFree(h_pin);
+  if (st) hipStreamDestroy(st);
+}
+
+void Engine::set_model(const Model& mm) {
+  int nroi = m.nroi;
+  m = mm;
+  m.nroi = nroi;
+}
+
+void Engine::upload_frames() {
+  DAO_CHECK(hipMemcpyAsync(d_fr, fr.data(), sizeof(Frame) * F, hipMemcpyHostToDevice, st));
+}
+
+void Engine::alloc_peaks(Peaks& p, long long c) {
+  p.cap = c;
+  p.prm = dalloc<double>(c * kNFit);
+  p.lam = dalloc<double>(c);
+  p.err = dalloc<double>(c);
+  p.xi = dalloc<int>(c);
+  p.yi = dalloc<int>(c);
+  p.status = dalloc<int>(c);
+  p.iter = dalloc<int>(c);
+  p.added = dalloc<int>(c);
+  p.frame = dalloc<int>(c);
+  p.bk_prm = dalloc<double>(c * kNFit);
+  p.bk_lam = dalloc<double>(c);
+  p.bk_err = dalloc<double>(c);
+  p.bk_xi = dalloc<int>(c);
+  p.bk_yi = dalloc<int>(c);
+  p.bk_status = dalloc<int>(c);
+  p.bk_iter = dalloc<int>(c);
+  p.bk_added = dalloc<int>(c);
+}
+
+void Engine::free_peaks(Peaks& p) {
+  dfree(p.prm);
+  dfree(p.lam);
+  dfree(p.err);
+  dfree(p.xi);
+  dfree(p.yi);
+  dfree(p.status);
+  dfree(p.iter);
+  dfree(p.added);
+  dfree(p.frame);
+  dfree(p.bk_prm);
+  dfree(p.bk_lam);
+  dfree(p.bk_err);
+  dfree(p.bk_xi);
+  dfree(p.bk_yi);
+  dfree(p.bk_status);
+  dfree(p.bk_iter);
+  dfree(p.bk_added);
+}
+
+void Engine::alloc_sched(long long c) {
+  sc.op_peak = dalloc<int>(c);
+  sc.box = dalloc<int4>(c);
+  sc.pred_n = dalloc<int>(c);
+  sc.pred = dalloc<int>(c * kMaxPred);
+  sc.done = dalloc<int>(c);
+  sc.cell_ops = dalloc<int>(c);
+  sc.big_ops = dalloc<int>(c);
+  sc.op_delta = dalloc<int>(c * C_COUNT);
+  sc.op_flags = dalloc<int>(c);
+  sc.esc_box = dalloc<int4>(c);
+  d_override = dalloc<int4>(c);
+  d_flag = dalloc<int>(c);
+}
+
+void Engine::free_sched() {
+  dfree(sc.op_peak);
+  dfree(sc.box);
+  dfree(sc.pred_n);
+  dfree(sc.pred);
+  dfree(sc.done);
+  dfree(sc.cell_ops);
+  dfree(sc.big_ops);
+  dfree(sc.op_delta);
+  dfree(sc.op_flags);
+  dfree(sc.esc_box);
+  dfree(d_override);
+  dfree(d_flag);
+}
+
+// Grow every per-slot array to c slots, keeping the first `cap` slots.
+void Engine::grow_total_cap(long long c) {
+  if (c <= cap) return;
+  Peaks np{};
+  alloc_peaks(np, c);
+  if (cap > 0) {
+    for (int q = 0; q < kNFit; q++)
+      DAO_CHECK(hipMemcpyAsync(np.prm + q * c, pk.prm + q * cap, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.lam, pk.lam, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.err, pk.err, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.xi, pk.xi, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.yi, pk.yi, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.status, pk.status, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.iter, pk.iter, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(np.added, pk.added, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
+  }
+  DAO_CHECK(hipStreamSynchronize(st));
+  free_peaks(pk);
+  free_peaks(pk2);
+  free_sched();
+  pk = np;
+  alloc_peaks(pk2, c);
+  alloc_sched(c);
+  cap = c;
+  // slot -> frame map: frame f owns [peak0, peak0 + cap_f); the single-frame
+  // handle owns everything
+  std::vector<int> owner(c, F - 1);
+  for (int f = 0; f < F; f++)
+    for (long long s = fr[f].peak0; s < (long long)fr[f].peak0 + fr[f].cap && s < c; s++) owner[s] = f;
+  DAO_CHECK(hipMemcpy(pk.frame, owner.data(), sizeof(int) * c, hipMemcpyHostToDevice));
+  DAO_CHECK(hipMemcpy(pk2.frame, owner.data(), sizeof(int) * c, hipMemcpyHostToDevice));
+}
+
+void Engine::ensure_frame_cap(int f, int need) {
+  if (F != 1) {
+    if (need > fr[f].cap) fail("frame peak capacity exceeded");
+    return;
+  }
+  if (need <= fr[0].cap && need <= cap) return;
+  long long c = std::max<long long>({(long long)need, 2 * cap, 512});
+  fr[0].cap = (int)c;
+  grow_total_cap(c);
+  upload_frames();
+}
+
+int* Engine::pin(size_t bytes) {
+  if (bytes > h_pin_bytes) {
+    if (h_pin) DAO_CHECK(hipHostFree(h_pin));
+    size_t b = std::max(bytes, (size_t)4096);
+    DAO_CHECK(hipHostMalloc((void**)&h_pin, b, hipHostMallocDefault));
+    h_pin_bytes = b;
+  }
+  return h_pin;
+}
+
+double* Engine::stage(size_t bytes) {
+  if (bytes > stage_bytes) {
+    dfree(d_stage);
+    d_stage = dalloc<double>((long long)((bytes + 7) / 8));
+    stage_bytes = bytes;
+  }
+  return d_stage;
+}
+
+int* Engine::istage(size_t bytes) {
+  if (bytes > istage_bytes) {
+    dfree(d_istage);
+    d_istage = dalloc<int>((long long)((bytes + 3) / 4));
+    istage_bytes = bytes;
+  }
+  return d_istage;
+}
+
+void Engine::sync() { DAO_CHECK(hipStreamSynchronize(st)); }
+
+// ---------------------------------------------------------------------------
+// image state
+// ---------------------------------------------------------------------------
+
+void Engine::set_calibration(int f, const double* rqe_h, const double* scmos_h) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  DAO_CHECK(hipMemcpyAsync(px.rqe + fr[f].pix0, rqe_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  DAO_CHECK(hipMemcpyAsync(px.scmos + fr[f].pix0, scmos_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  sync();
+}
+
+void Engine::new_image(int f, const double* img) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  double* d = stage(sizeof(double) * n);
+  DAO_CHECK(hipMemcpyAsync(d, img, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  new_image_dev(f, d);
+  sync();
+}
+
+void Engine::new_image_dev(int f, const double* d) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  launch_new_image(px, fr[f].pix0, n, d, st);
+  fr[f].nfit = 0;
+  upload_frames();
+}
+
+void Engine::new_background(int f, const double* bg) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  double* d = stage(sizeof(double) * n);
+  DAO_CHECK(hipMemcpyAsync(d, bg, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  new_background_dev(f, d);
+  sync();
+}
+
+void Engine::new_background_dev(int f, const double* d) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  launch_new_background(px, fr[f].pix0, n, d, st);
+}
+
+void Engine::anscombe(int f) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  launch_anscombe(px, fr[f].pix0, n, st);
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// the ordered executor
+// ---------------------------------------------------------------------------
+
+OpArgs Engine::args(int round) {
+  OpArgs a;
+  a.m = m;
+  a.px = px;
+  a.pk = pk;
+  a.sc = sc;
+  a.frames = d_fr;
+  a.roi_tab = d_roi;
+  a.round = round;
+  return a;
+}
+
+// Run the ops sc.op_peak[0..nops) (boxes already built) to completion.
+void Engine::run_ops(int kind, const double* rows, int fmt, int first_slot) {
+  if (sc.nops == 0) return;
+  launch_fill_i32(sc.n_big, 1, 0, st);
+  launch_binning(sc, d_fr, pk, ncells, st);
+  launch_fill_i32(sc.remaining, 1, sc.nops, st);
+  int* h = pin(sizeof(int));
+  int round = 0;
+  int group = 4;
+  int last = sc.nops;
+  for (;;) {
+    for (int r = 0; r < group; r++) launch_op_round(kind, args(round++), st, rows, fmt, first_slot);
+    DAO_CHECK(hipMemcpyAsync(h, sc.remaining, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    int rem = h[0];
+    if (rem == 0) break;
+    if (rem >= last) fail("ordered executor made no progress");
+    last = rem;
+    group = std::min(group * 2, 32);
+  }
+  rounds_total += round;
+}
+
+int Engine::select(int mode) {
+  launch_select(pk, d_fr, cap, mode, sc.frame_bad, d_flag, sc.op_peak, d_count, st);
+  int* h = pin(sizeof(int));
+  DAO_CHECK(hipMemcpyAsync(h, d_count, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync();
+  return h[0];
+}
+
+// mFitIterateLM (multi_fit.c:1072-1349) for every frame of the batch.
+void Engine::sweep() {
+  launch_fill_i32((int*)d_override, cap * 4, 0, st);
+  launch_fill_i32(sc.frame_bad, F, 0, st);
+  sc.nops = select(SEL_RUNNING);
+  int attempt = 0;
+  while (sc.nops > 0) {
+    launch_box(OP_LM, sc, pk, d_fr, m, d_override, nullptr, 0, 0, st);
+    launch_fill_i32(sc.frame_bad, F, 0, st);
+    run_ops(OP_LM, nullptr, 0, 0);
+    launch_commit(sc, pk, d_counters, st);
+    int* h = pin(sizeof(int) * F);
+    DAO_CHECK(hipMemcpyAsync(h, sc.frame_bad, sizeof(int) * F, hipMemcpyDeviceToHost, st));
+    sync();
+    int bad = 0;
+    for (int f = 0; f < F; f++) bad += h[f] ? 1 : 0;
+    if (bad == 0) break;
+    escapes_total += bad;
+    OpArgs a = args(0);
+    launch_rollback(a, st);
+    launch_collect_escapes(sc, pk, d_override, st);
+    attempt++;
+    max_attempts_seen = std::max(max_attempts_seen, attempt);
+    if (attempt > 64) fail("sweep did not settle after 64 attempts");
+    sc.nops = select(SEL_RUNNING_BAD);
+  }
+  // mFitRecenterPeaks (multi_fit.c:1644-1690)
+  launch_fill_i32(sc.frame_bad, F, 0, st);
+  sc.nops = select(SEL_RECENTER);
+  if (sc.nops > 0) {
+    launch_box(OP_RECENTER, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
+    run_ops(OP_RECENTER, nullptr, 0, 0);
+    launch_commit(sc, pk, d_counters, st);
+  }
+}
+
+// daoNewPeaks (dao_fit.c:1582-1766) for one frame; storage policy is the
+// caller's (the ABI mirrors mFitNewPeaks, multi_fit.c:1419-1499).
+void Engine::new_peaks(int f, const double* rows, int fmt, int n) {
+  if (n <= 0) return;
+  int width = fmt == PF_TEXT ? 7 : 4;
+  ensure_frame_cap(f, fr[f].nfit + n);
+  double* d_rows = stage(sizeof(double) * (size_t)n * width);
+  DAO_CHECK(hipMemcpyAsync(d_rows, rows, sizeof(double) * (size_t)n * width, hipMemcpyHostToDevice, st));
+  int first = fr[f].peak0 + fr[f].nfit;
+  launch_iota(sc.op_peak, n, first, st);
+  sc.nops = n;
+  launch_fill_i32(sc.frame_bad, F, 0, st);
+  launch_box(OP_NEWPEAK, sc, pk, d_fr, m, nullptr, d_rows, fmt, first, st);
+  run_ops(OP_NEWPEAK, d_rows, fmt, first);
+  launch_commit(sc, pk, d_counters, st);
+  fr[f].nfit += n;
+  upload_frames();
+  // initial errors, in peak order
+  launch_box(OP_PEAKERROR, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
+  run_ops(OP_PEAKERROR, nullptr, 0, 0);
+  launch_commit(sc, pk, d_counters, st);
+  sync();
+}
+
+// Batch form: new peaks for several frames in one executor pass.  rows_dev[k]
+// are device pointers; all frames use the same format.
+void Engine::new_peaks_multi(const std::vector<int>& frames, const std::vector<const double*>& rows_dev,
+                             const std::vector<int>& counts, int fmt) {
+  int width = fmt == PF_TEXT ? 7 : 4;
+  long long total = 0;
+  for (size_t k = 0; k < frames.size(); k++) total += counts[k];
+  if (total == 0) return;
+  // concatenate rows in op order (frames ascending)
+  std::vector<size_t> order(frames.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return frames[a] < frames[b]; });
+  double* d_rows = stage(sizeof(double) * (size_t)total * width);
+  int* ops = istage(sizeof(int) * (size_t)total);
+  long long at = 0;
+  for (size_t k : order) {
+    int f = frames[k], n = counts[k];
+    if (n == 0) continue;
+    if (fr[f].nfit + n > fr[f].cap) fail("frame peak capacity exceeded");
+    DAO_CHECK(hipMemcpyAsync(d_rows + at * width, rows_dev[k], sizeof(double) * (size_t)n * width,
+                             hipMemcpyDeviceToDevice, st));
+    launch_iota(ops + at, n, fr[f].peak0 + fr[f].nfit, st);
+    at += n;
+  }
+  DAO_CHECK(hipMemcpyAsync(sc.op_peak, ops, sizeof(int) * total, hipMemcpyDeviceToDevice, st));
+  sc.nops = (int)total;
+  launch_fill_i32(sc.frame_bad, F, 0, st);
+  launch_box(OP_NEWPEAK, sc, pk, d_fr, m, nullptr, d_rows, fmt, 0, st);
+  run_ops(OP_NEWPEAK, d_rows, fmt, 0);
+  launch_commit(sc, pk, d_counters, st);
+  for (size_t k = 0; k < frames.size(); k++) fr[frames[k]].nfit += counts[k];
+  upload_frames();
+  launch_box(OP_PEAKERROR, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
+  run_ops(OP_PEAKERROR, nullptr, 0, 0);
+  launch_commit(sc, pk, d_counters, st);
+}
+
+// Stable compaction of frame f's live peaks to the slots listed in keep.
+void Engine::compact(int f, const std::vector<int32_t>& keep) {
+  int n = (int)keep.size();
+  if (n == fr[f].nfit) {
+    bool same = true;
+    for (int i = 0; i < n && same; i++) same = keep[i] == i;
+    if (same) return;
+  }
+  if (n > 0) {
+    int* d_map = istage(sizeof(int) * n);
+    DAO_CHECK(hipMemcpyAsync(d_map, keep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    launch_gather_peaks(pk2, pk, d_map, fr[f].peak0, n, st);
+    // copy the compacted range back into pk (other frames' slots untouched)
+    long long s0 = fr[f].peak0;
+    for (int q = 0; q < kNFit; q++)
+      DAO_CHECK(hipMemcpyAsync(pk.prm + q * cap + s0, pk2.prm + q * cap + s0, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.lam + s0, pk2.lam + s0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.err + s0, pk2.err + s0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.xi + s0, pk2.xi + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.yi + s0, pk2.yi + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.status + s0, pk2.status + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.iter + s0, pk2.iter + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(pk.added + s0, pk2.added + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+  }
+  fr[f].nfit = n;
+  upload_frames();
+  sync();
+}
+
+// mFitSetPeakStatus (multi_fit.c:1809-1831).
+void Engine::set_status(int f, const int32_t* status) {
+  int n = fr[f].nfit;
+  if (n == 0) return;
+  std::vector<int32_t> added;
+  get_int(f, pk.added, added);
+  std::vector<int> ops;
+  for (int i = 0; i < n; i++)
+    if (status[i] == ST_ERROR && added[i] > 0) ops.push_back(fr[f].peak0 + i);
+  if (!ops.empty()) {
+    int* d = istage(sizeof(int) * ops.size());
+    DAO_CHECK(hipMemcpyAsync(d, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, st));
+    DAO_CHECK(hipMemcpyAsync(sc.op_peak, d, sizeof(int) * ops.size(), hipMemcpyDeviceToDevice, st));
+    sc.nops = (int)ops.size();
+    launch_fill_i32(sc.frame_bad, F, 0, st);
+    launch_box(OP_SUBTRACT, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
+    run_ops(OP_SUBTRACT, nullptr, 0, 0);
+  }
+  int* d = istage(sizeof(int) * n);
+  DAO_CHECK(hipMemcpyAsync(d, status, sizeof(int) * n, hipMemcpyHostToDevice, st));
+  launch_set_status(pk, fr[f].peak0, n, d, st);
+  sync();
+}
+
+// ---------------------------------------------------------------------------
+// read-back
+// ---------------------------------------------------------------------------
+
+void Engine::read_counters(std::vector<int>& out) {
+  size_t n = (size_t)F * C_COUNT;
+  int* h = pin(sizeof(int) * n);
+  DAO_CHECK(hipMemcpyAsync(h, d_counters, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  launch_fill_i32(d_counters, (long long)n, 0, st);
+  sync();
+  out.assign(h, h + n);
+}
+
+void Engine::get_int(int f, const int* arr, std::vector<int32_t>& out) {
+  int n = fr[f].nfit;
+  out.resize(n);
+  if (n == 0) return;
+  DAO_CHECK(hipMemcpyAsync(out.data(), arr + fr[f].peak0, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Engine::get_status(int f, std::vector<int32_t>& out) { get_int(f, pk.status, out); }
+
+void Engine::get_double(int f, const double* arr, std::vector<double>& out) {
+  int n = fr[f].nfit;
+  out.resize(n);
+  if (n == 0) return;
+  DAO_CHECK(hipMemcpyAsync(out.data(), arr + fr[f].peak0, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Engine::get_param(int f, int q, std::vector<double>& out) { get_double(f, pk.prm + (long long)q * cap, out); }
+
+void Engine::get_sums(int f, int what, std::vector<double>& out) {
+  int n = fr[f].nfit;
+  out.resize(n);
+  if (n == 0) return;
+  double* d = stage(sizeof(double) * n);
+  launch_peak_sums(m, px, pk, d_fr, d_roi, fr[f].peak0, n, what, d, st);
+  DAO_CHECK(hipMemcpyAsync(out.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Engine::get_jacobian(int f, std::vector<double>& out) {
+  int n = fr[f].nfit;
+  out.resize((size_t)n * m.jac);
+  if (n == 0) return;
+  double* d = stage(sizeof(double) * n * m.jac);
+  launch_peak_jacobian(m, px, pk, d_fr, d_roi, fr[f].peak0, n, d, st);
+  DAO_CHECK(hipMemcpyAsync(out.data(), d, sizeof(double) * n * m.jac, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Engine::get_image(int f, double* out, int residual) {
+  long long n = (long long)fr[f].W * fr[f].H;
+  double* d = stage(sizeof(double) * n);
+  launch_fit_image(px, fr[f].pix0, n, d, residual, st);
+  DAO_CHECK(hipMemcpyAsync(out, d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+int Engine::count_status(int f, int status) {
+  std::vector<int32_t> s;
+  get_status(f, s);
+  int c = 0;
+  for (int v : s) c += (v == status) ? 1 : 0;
+  return c;
+}
+
+}  // namespace dao

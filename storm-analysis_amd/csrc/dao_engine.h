@@ -1,0 +1,133 @@
+// Host runtime of the MI355X fitter: owns the device state of a batch of
+// frames and drives the HIP kernels.  The single-frame C-ABI (dao_fit_abi.cpp)
+// and the batch / serving API (dao_batch.cpp) are both built on it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "dao_device.h"
+
+#define DAO_CHECK(expr)                                                                       \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) dao::fatal(__FILE__, __LINE__, #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+namespace dao {
+
+[[noreturn]] void fatal(const char* file, int line, const char* what, const char* msg);
+
+// kernel launchers (dao_kernels.hip)
+
+void launch_op_round(int kind, const OpArgs& a, hipStream_t st, const double* rows, int fmt, int first_slot);
+void launch_rollback(const OpArgs& a, hipStream_t st);
+void launch_binning(const Sched& sc, const Frame* frames, const Peaks& pk, int ncells, hipStream_t st);
+void launch_box(int kind, const Sched& sc, const Peaks& pk, const Frame* frames, const Model& m,
+                const int4* override_box, const double* rows, int fmt, int first_slot, hipStream_t st);
+void launch_commit(const Sched& sc, const Peaks& pk, int* frame_counters, hipStream_t st);
+void launch_collect_escapes(const Sched& sc, const Peaks& pk, int4* override_box, hipStream_t st);
+void launch_new_image(const Pix& px, long long pix0, long long n, const double* img, hipStream_t st);
+void launch_new_background(const Pix& px, long long pix0, long long n, const double* bg, hipStream_t st);
+void launch_anscombe(const Pix& px, long long pix0, long long n, hipStream_t st);
+void launch_fit_image(const Pix& px, long long pix0, long long n, double* out, int residual, hipStream_t st);
+void launch_peak_sums(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames, const int* roi_tab,
+                      int slot0, int n, int what, double* out, hipStream_t st);
+void launch_peak_jacobian(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames,
+                          const int* roi_tab, int slot0, int n, double* out, hipStream_t st);
+void launch_gather_peaks(const Peaks& dst, const Peaks& src, const int* map, int slot0, int n, hipStream_t st);
+void launch_set_status(const Peaks& pk, int slot0, int n, const int* status, hipStream_t st);
+void launch_select(const Peaks& pk, const Frame* frames, long long nslots, int mode, const int* frame_bad,
+                   int* flag, int* out, int* count, hipStream_t st);
+void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
+void launch_iota(int* p, int n, int base, hipStream_t st);
+
+// Reference ROI table (multi_fit.c:1031-1060): pixels whose centre lies within
+// r/2 of the ROI centre, row-major.
+void roi_table(int roi, std::vector<int>& rx, std::vector<int>& ry);
+
+struct FrameSpec {
+  int W, H;
+  int cap;  // peak slots reserved for the frame
+};
+
+class Engine {
+ public:
+  Engine(const Model& m, const std::vector<FrameSpec>& frames);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  Model m;
+  hipStream_t st = nullptr;
+  int F = 0;
+  std::vector<Frame> fr;
+  Frame* d_fr = nullptr;
+  Pix px{};
+  long long npix = 0;
+  Peaks pk{}, pk2{};
+  long long cap = 0;  // total peak slots
+  std::vector<int> roi_x, roi_y;
+  int* d_roi = nullptr;
+  Sched sc{};
+  int ncells = 0;
+  int4* d_override = nullptr;
+  int* d_counters = nullptr;  // [F][C_COUNT]
+  int* d_flag = nullptr;
+  int* d_count = nullptr;
+  int* h_pin = nullptr;  // pinned scratch for small read-backs
+  size_t h_pin_bytes = 0;
+  double* d_stage = nullptr;  // upload staging
+  size_t stage_bytes = 0;
+  int* d_istage = nullptr;
+  size_t istage_bytes = 0;
+  int max_attempts_seen = 0;
+  long long escapes_total = 0;
+  long long rounds_total = 0;
+
+  void set_model(const Model& mm);
+  void upload_frames();
+  // image state
+  void new_image(int f, const double* img_host);
+  void new_image_dev(int f, const double* img_dev);
+  void new_background(int f, const double* bg_host);
+  void new_background_dev(int f, const double* bg_dev);
+  void anscombe(int f);
+  void set_calibration(int f, const double* rqe_host, const double* scmos_host);
+  // peaks
+  void ensure_frame_cap(int f, int need);  // single-frame growth (F == 1 only)
+  void new_peaks(int f, const double* rows_host, int fmt, int n);
+  void new_peaks_multi(const std::vector<int>& frames, const std::vector<const double*>& rows_dev,
+                       const std::vector<int>& counts, int fmt);
+  void compact(int f, const std::vector<int32_t>& keep);
+  void set_status(int f, const int32_t* status);
+  // fitting
+  void sweep();  // one mFitIterateLM over every frame of the batch
+  // read-back
+  void read_counters(std::vector<int>& out);  // F * C_COUNT, zeroes the device copy
+  void get_status(int f, std::vector<int32_t>& out);
+  void get_int(int f, const int* arr, std::vector<int32_t>& out);
+  void get_double(int f, const double* arr, std::vector<double>& out);
+  void get_param(int f, int q, std::vector<double>& out);
+  void get_sums(int f, int what, std::vector<double>& out);
+  void get_jacobian(int f, std::vector<double>& out);
+  void get_image(int f, double* out_host, int residual);
+  int count_status(int f, int status);
+  void sync();
+
+ private:
+  void alloc_peaks(Peaks& p, long long c);
+  void free_peaks(Peaks& p);
+  void alloc_sched(long long c);
+  void free_sched();
+  void grow_total_cap(long long c);
+  int* pin(size_t bytes);
+  double* stage(size_t bytes);
+  int* istage(size_t bytes);
+  int select(int mode);
+  void run_ops(int kind, const double* rows, int fmt, int first_slot);
+  OpArgs args(int round);
+};
+
+}  // namespace dao

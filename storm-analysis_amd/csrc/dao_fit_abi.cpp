@@ -1,0 +1,306 @@
+// C-ABI of libdao_fit.so: the entry points of the reference's libdao_fit
+// (bound by storm_analysis/sa_library/dao_fit_c.py:147-238), each returning
+// only when its GPU work is done.  The handle is a host struct whose leading
+// fields have the reference's fitData layout (multi_fit.h:80-143) so that the
+// Python surface can read the counters, nfit, max_nfit, jac_size and the ROI
+// tables directly; the device state lives behind it in a dao::Engine.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "dao_engine.h"
+#include "../../include/dao_fit_amd.h"
+
+using dao::Engine;
+
+namespace {
+
+constexpr int kPeakBlock = 500;  // INCNPEAKS (multi_fit.h:49)
+
+struct Handle {
+  fitData pub;  // must stay first
+  std::unique_ptr<Engine> eng;
+  dao::Model model;
+  std::vector<int> roi_x, roi_y;
+};
+
+Handle* H(fitData* fd) { return reinterpret_cast<Handle*>(fd); }
+
+void add_counters(Handle* h) {
+  std::vector<int> c;
+  h->eng->read_counters(c);
+  fitData& p = h->pub;
+  p.n_dposv += c[dao::C_DPOSV];
+  p.n_iterations += c[dao::C_ITER];
+  p.n_lost += c[dao::C_LOST];
+  p.n_margin += c[dao::C_MARGIN];
+  p.n_neg_fi += c[dao::C_NEG_FI];
+  p.n_neg_height += c[dao::C_NEG_HEIGHT];
+  p.n_non_converged += c[dao::C_NON_CONV];
+  p.n_non_decr += c[dao::C_NON_DECR];
+}
+
+void set_model(Handle* h, int shape, int errm, int jac) {
+  h->model.shape = shape;
+  h->model.errm = errm;
+  h->model.jac = jac;
+  h->pub.jac_size = jac;
+  h->eng->set_model(h->model);
+}
+
+}  // namespace
+
+extern "C" {
+
+fitData* daoInitialize(double* rqe, double* scmos_cal, double tol, int im_size_x, int im_size_y, int roi_size) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    std::fprintf(stderr, "dao_fit (HIP): no GPU device is available\n");
+    return nullptr;
+  }
+  if (roi_size < 1 || roi_size > 32 || im_size_x < 1 || im_size_y < 1) {
+    std::fprintf(stderr, "dao_fit (HIP): unsupported sizes (roi %d, image %d x %d)\n", roi_size, im_size_x,
+                 im_size_y);
+    return nullptr;
+  }
+  Handle* h = new Handle();
+  std::memset(&h->pub, 0, sizeof(fitData));
+  fitData& p = h->pub;
+  p.image_size_x = im_size_x;
+  p.image_size_y = im_size_y;
+  p.tolerance = tol;
+  p.minimum_height = -1.0;
+  p.fit_size_x = roi_size;
+  p.fit_size_y = roi_size;
+  p.xoff = 0.5 * (double)roi_size - 1.0;
+  p.yoff = 0.5 * (double)roi_size - 1.0;
+  p.zoff = 0.0;
+
+  dao::Model& m = h->model;
+  std::memset(&m, 0, sizeof(m));
+  m.shape = dao::SH_2DFIXED;
+  m.errm = dao::EM_MLE;
+  m.roi = roi_size;
+  m.jac = 4;
+  m.margin = 2;
+  m.xoff = p.xoff;
+  m.zoff = 0.0;
+  m.tol = tol;
+  m.min_height = -1.0;
+
+  dao::roi_table(roi_size, h->roi_x, h->roi_y);
+  p.roi_n_index = (int)h->roi_x.size();
+  p.roi_x_index = h->roi_x.data();
+  p.roi_y_index = h->roi_y.data();
+
+  std::vector<dao::FrameSpec> spec(1);
+  spec[0].W = im_size_x;
+  spec[0].H = im_size_y;
+  spec[0].cap = 0;
+  h->eng.reset(new Engine(m, spec));
+  h->eng->set_calibration(0, rqe, scmos_cal);
+  return &h->pub;
+}
+
+void daoInitialize2DFixed(fitData* fd) { set_model(H(fd), dao::SH_2DFIXED, dao::EM_MLE, 4); }
+void daoInitialize2DFixedALS(fitData* fd) { set_model(H(fd), dao::SH_2DFIXED, dao::EM_ALS, 4); }
+void daoInitialize2DFixedLS(fitData* fd) { set_model(H(fd), dao::SH_2DFIXED, dao::EM_LS, 4); }
+void daoInitialize2DFixedDWLS(fitData* fd) { set_model(H(fd), dao::SH_2DFIXED, dao::EM_DWLS, 4); }
+void daoInitialize2DFixedFWLS(fitData* fd) { set_model(H(fd), dao::SH_2DFIXED, dao::EM_FWLS, 4); }
+
+static void init_2d(fitData* fd, int errm, double wmin, double wmax) {
+  Handle* h = H(fd);
+  h->model.wmin = wmin;
+  h->model.wmax = wmax;
+  set_model(h, dao::SH_2D, errm, 5);
+}
+void daoInitialize2D(fitData* fd, double a, double b) { init_2d(fd, dao::EM_MLE, a, b); }
+void daoInitialize2DALS(fitData* fd, double a, double b) { init_2d(fd, dao::EM_ALS, a, b); }
+void daoInitialize2DLS(fitData* fd, double a, double b) { init_2d(fd, dao::EM_LS, a, b); }
+void daoInitialize2DDWLS(fitData* fd, double a, double b) { init_2d(fd, dao::EM_DWLS, a, b); }
+void daoInitialize2DFWLS(fitData* fd, double a, double b) { init_2d(fd, dao::EM_FWLS, a, b); }
+
+void daoInitialize3D(fitData* fd, double wmin, double wmax) {
+  Handle* h = H(fd);
+  h->model.wmin = wmin;
+  h->model.wmax = wmax;
+  set_model(h, dao::SH_3D, dao::EM_MLE, 6);
+}
+
+void daoInitializeZ(fitData* fd, double* wx, double* wy, double z_min, double z_max) {
+  Handle* h = H(fd);
+  for (int i = 0; i < 5; i++) {
+    h->model.wxz[i] = wx[i];
+    h->model.wyz[i] = wy[i];
+  }
+  h->model.wxz[0] = h->model.wxz[0] * h->model.wxz[0];
+  h->model.wyz[0] = h->model.wyz[0] * h->model.wyz[0];
+  h->model.zmin = z_min;
+  h->model.zmax = z_max;
+  set_model(h, dao::SH_Z, dao::EM_MLE, 5);
+}
+
+void daoNewPeaks(fitData* fd, double* peak_params, char* p_type, int n_peaks) {
+  Handle* h = H(fd);
+  fitData& p = h->pub;
+  int fmt;
+  if (!std::strcmp(p_type, "finder")) {
+    fmt = dao::PF_FINDER;
+  } else if (!std::strcmp(p_type, "testing")) {
+    fmt = dao::PF_TESTING;
+  } else if (!std::strcmp(p_type, "text") || !std::strcmp(p_type, "hdf5")) {
+    fmt = dao::PF_TEXT;
+  } else {
+    fmt = -1;
+  }
+  // storage policy of mFitNewPeaks (multi_fit.c:1428-1477): growing drops
+  // ERROR peaks and rounds the capacity to blocks of 500
+  if (p.nfit + n_peaks > p.max_nfit) {
+    int n_alloc = kPeakBlock * ((p.nfit + n_peaks) / kPeakBlock + 1);
+    std::vector<int32_t> st, keep;
+    h->eng->get_status(0, st);
+    for (int i = 0; i < (int)st.size(); i++)
+      if (st[i] != dao::ST_ERROR) keep.push_back(i);
+    h->eng->compact(0, keep);
+    p.max_nfit = n_alloc;
+    p.nfit = (int)keep.size();
+  }
+  if (fmt < 0) {
+    // the reference allocates and initialises the slots, then leaves them
+    // untouched (dao_fit.c:1753-1755)
+    std::printf("Unrecognized peak type '%s'!\n", p_type);
+    return;
+  }
+  if (n_peaks > 0) {
+    h->eng->new_peaks(0, peak_params, fmt, n_peaks);
+    p.nfit += n_peaks;
+  }
+  add_counters(h);
+}
+
+void daoCleanup(fitData* fd) {
+  if (fd == nullptr) return;
+  Handle* h = H(fd);
+  h->eng.reset();
+  delete h;
+}
+
+void mFitNewImage(fitData* fd, double* image) {
+  Handle* h = H(fd);
+  h->eng->new_image(0, image);
+  h->pub.nfit = 0;
+}
+
+void mFitNewBackground(fitData* fd, double* background) { H(fd)->eng->new_background(0, background); }
+
+void mFitAnscombeTransformImage(fitData* fd) { H(fd)->eng->anscombe(0); }
+
+void mFitIterateLM(fitData* fd) {
+  Handle* h = H(fd);
+  h->eng->sweep();
+  add_counters(h);
+}
+
+int mFitGetUnconverged(fitData* fd) { return H(fd)->eng->count_status(0, dao::ST_RUNNING); }
+
+int mFitGetNError(fitData* fd) { return H(fd)->eng->count_status(0, dao::ST_ERROR); }
+
+void mFitGetFitImage(fitData* fd, double* out) { H(fd)->eng->get_image(0, out, 0); }
+
+void mFitGetResidual(fitData* fd, double* out) { H(fd)->eng->get_image(0, out, 1); }
+
+void mFitGetPeakPropertyDouble(fitData* fd, double* values, char* what) {
+  Handle* h = H(fd);
+  Engine& e = *h->eng;
+  const fitData& p = h->pub;
+  std::vector<double> v;
+  int n = p.nfit;
+  auto put = [&](double off) {
+    for (int i = 0; i < n; i++) values[i] = v[i] + off;
+  };
+  if (!std::strcmp(what, "background")) {
+    e.get_param(0, dao::P_BG, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "bg_sum")) {
+    e.get_sums(0, 0, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "error")) {
+    e.get_double(0, e.pk.err, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "fg_sum") || !std::strcmp(what, "fg_sum_sc")) {
+    e.get_sums(0, 1, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "height")) {
+    e.get_param(0, dao::P_HEIGHT, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "jacobian")) {
+    e.get_jacobian(0, v);
+    for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
+  } else if (!std::strcmp(what, "sum")) {
+    e.get_sums(0, 2, v);
+    put(0.0);
+  } else if (!std::strcmp(what, "x")) {
+    e.get_param(0, dao::P_XC, v);
+    put(p.xoff);
+  } else if (!std::strcmp(what, "xsigma")) {
+    e.get_param(0, dao::P_XW, v);
+    for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
+  } else if (!std::strcmp(what, "y")) {
+    e.get_param(0, dao::P_YC, v);
+    put(p.yoff);
+  } else if (!std::strcmp(what, "ysigma")) {
+    e.get_param(0, dao::P_YW, v);
+    for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
+  } else if (!std::strcmp(what, "z")) {
+    e.get_param(0, dao::P_ZC, v);
+    put(p.zoff);
+  } else {
+    std::printf("Unrecognized parameter '%s'!\n", what);
+  }
+}
+
+void mFitGetPeakPropertyInt(fitData* fd, int32_t* values, char* what) {
+  Handle* h = H(fd);
+  std::vector<int32_t> v;
+  if (!std::strcmp(what, "iterations")) {
+    h->eng->get_int(0, h->eng->pk.iter, v);
+  } else if (!std::strcmp(what, "status")) {
+    h->eng->get_status(0, v);
+  } else {
+    std::printf("Unrecognized parameter '%s'!\n", what);
+    return;
+  }
+  for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
+}
+
+void mFitRemoveErrorPeaks(fitData* fd) {
+  Handle* h = H(fd);
+  std::vector<int32_t> st, keep;
+  h->eng->get_status(0, st);
+  for (int i = 0; i < (int)st.size(); i++)
+    if (st[i] != dao::ST_ERROR) keep.push_back(i);
+  h->eng->compact(0, keep);
+  h->pub.nfit = (int)keep.size();
+}
+
+void mFitRemoveRunningPeaks(fitData* fd) {
+  Handle* h = H(fd);
+  std::vector<int32_t> st, keep;
+  h->eng->get_status(0, st);
+  for (int i = 0; i < (int)st.size(); i++) {
+    if (st[i] != dao::ST_RUNNING)
+      keep.push_back(i);
+    else
+      h->pub.n_non_converged++;
+  }
+  h->eng->compact(0, keep);
+  h->pub.nfit = (int)keep.size();
+}
+
+void mFitSetPeakStatus(fitData* fd, int32_t* status) { H(fd)->eng->set_status(0, status); }
+
+}  // extern "C"
