@@ -1,0 +1,261 @@
+#!/usr/bin/env python
+"""
+Benchmark: converged emitter fits/s of the MI355X 3D-DAOSTORM fitter, batched
+vs plain, with frame latency, on the 3D-astigmatism workload: 500 frames of
+512 x 512, 1000 emitters per frame, "Z" model (wx = [2.0 px, +0.15 um, 0.4 um,
+0, 0], wy = [2.0, -0.15, 0.4, 0, 0], z in [-0.5, 0.5] um), ROI 18 px, MLE,
+FP64 state, tolerance 1e-6.
+
+One step = newImage + newBackground + newPeaks ("hdf5" start peaks) + doFit
+(until no peak is RUNNING, at most 2000 sweeps) + all peak properties back on
+the host, for every frame of the batch; images and start peaks are resident
+in HBM when the timed region starts.  `value` is batched converged fits/s.
+Also reported: the plain path (MultiFitterZ, one frame at a time through the
+single-frame C-ABI, host arrays in and out) with its p50/p99 frame latency,
+the LM kernel's HBM roofline (algorithmic bytes = cycles x N_roi x 36 B over
+the LM kernel's event-timed duration), and the oracle timed on the host's
+cores on a sample of the same frames.
+"""
+import argparse
+import concurrent.futures as cf
+import ctypes
+import json
+import os
+import sys
+import time
+from unittest import mock
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+ROI = 18
+N_ROI_18 = 256
+ALG_BYTES_PER_PIXEL = 36
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--frames", type=int, default=500)
+    p.add_argument("--emitters", type=int, default=1000)
+    p.add_argument("--size", type=int, default=512)
+    p.add_argument("--plain-frames", type=int, default=20)
+    p.add_argument("--cpu-frames", type=int, default=32)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--margin", type=int, default=2)
+    return p.parse_args()
+
+
+def zkw():
+    from storm_analysis_amd.diagnostics import synth
+    return {"wx_params": synth.WX_Z, "wy_params": synth.WY_Z, "min_z": -0.5, "max_z": 0.5}
+
+
+def frame_rows(rows, counts, f):
+    c0 = int(counts[:f].sum())
+    return rows[c0:c0 + int(counts[f])]
+
+
+def peaks_dict(r):
+    return {"x": r[:, 0], "y": r[:, 1], "z": r[:, 2], "background": r[:, 3], "height": r[:, 4],
+            "xsigma": r[:, 5], "ysigma": r[:, 6]}
+
+
+def fit_plain(cls_factory, image, background, rows):
+    m = cls_factory()
+    m.initializeC(image)
+    m.newImage(image)
+    m.newBackground(background)
+    m.newPeaks(peaks_dict(rows), "hdf5")
+    m.doFit()
+    out = {k: m.getPeakProperty(k) for k in ("x", "y", "z", "height", "background", "status")}
+    m.cleanup(verbose=False)
+    return out
+
+
+def cpu_baseline(images, backgrounds, rows, counts, nframes):
+    """The oracle (FP64 C restatement) on the host's cores, one fitter per frame."""
+    from storm_analysis_amd.sa_library import dao_fit_c as D
+    lib = D.setDaoFitArgtypes(ctypes.cdll.LoadLibrary(os.path.join(ROOT, "oracle", "liboracle_dao.so")))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+
+    def one(f):
+        with mock.patch.object(D, "loadDaoFitC", lambda: lib):
+            fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
+        out = fit_plain(fac, images[f], backgrounds[f], frame_rows(rows, counts, f))
+        return int((out["status"] == D.CONVERGED).sum()), out
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        res = list(ex.map(one, range(nframes)))
+    dt = time.perf_counter() - t0
+    conv = sum(r[0] for r in res)
+    return {"value": conv / dt, "unit": "converged fits/s", "cores": cores, "kind": "port",
+            "sample": "%d of the workload's frames (%d converged fits) in %.1f s on %d threads" %
+                      (nframes, conv, dt, cores)}, [r[1] for r in res]
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the fitter has no multi-GPU path (frames would be "
+              "sharded across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    __graft_entry__.load_package()
+    from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
+    from storm_analysis_amd.diagnostics import synth_gpu
+
+    dev = "cuda:0"
+    F, S, N = args.frames, args.size, args.emitters
+    t_gen = time.perf_counter()
+    log("generating %d frames of %dx%d with %d emitters on %s" % (F, S, S, N, dev))
+    images, backgrounds, rows, counts, truth = synth_gpu.frames_z(F, size=S, n=N, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+
+    bf = dao_batch.BatchFitter([(S, S)] * F, [N] * F, model="Z", roi_size=ROI, margin=args.margin, **zkw())
+    counts_np = counts.numpy()
+
+    def step():
+        bf.newImages(images)
+        bf.newBackgrounds(backgrounds)
+        bf.newPeaks(rows, "hdf5", counts=counts_np)
+        bf.doFit()
+        return bf.getAllPeaks()
+
+    log("batch of %d frames created; warmup %d step(s)" % (F, args.warmup))
+    for _ in range(args.warmup):
+        step()
+    log("timing %d step(s)" % args.steps)
+    for f in range(F):
+        bf.getCounters(f)
+    bf.setTiming(True)
+    s0 = bf.stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    conv = 0
+    for _ in range(args.steps):
+        out, status, per_frame = step()
+        conv += int((status == D.CONVERGED).sum())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s1 = bf.stats()
+    cycles = 0
+    escapes = 0
+    for f in range(F):
+        c = bf.getCounters(f)
+        cycles += c["n_iterations"]
+    batched_fps = conv / dt
+    lm_ms = s1["lm_ms"] - s0["lm_ms"]
+    lm_launches = s1["lm_launches"] - s0["lm_launches"]
+    alg_bytes = float(cycles) * N_ROI_18 * ALG_BYTES_PER_PIXEL
+    achieved = alg_bytes / (lm_ms * 1e-3) / 1e9 if lm_ms > 0 else 0.0
+
+    log("batched: %.1f ms/step, %d converged per step" % (dt * 1e3 / args.steps, conv // max(1, args.steps)))
+    # plain path: one frame at a time through the single-frame C-ABI
+    host_images = images[:args.plain_frames].cpu().numpy()
+    host_bgs = backgrounds[:args.plain_frames].cpu().numpy()
+    host_rows = rows.cpu().numpy()
+    fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
+    fit_plain(fac, host_images[0], host_bgs[0], frame_rows(host_rows, counts_np, 0))  # warm
+    lat = []
+    plain_conv = 0
+    plain_out = []
+    t1 = time.perf_counter()
+    for f in range(args.plain_frames):
+        ta = time.perf_counter()
+        o = fit_plain(fac, host_images[f], host_bgs[f], frame_rows(host_rows, counts_np, f))
+        lat.append(time.perf_counter() - ta)
+        plain_conv += int((o["status"] == D.CONVERGED).sum())
+        plain_out.append(o)
+    plain_dt = time.perf_counter() - t1
+    plain_fps = plain_conv / plain_dt
+    lat_ms = numpy.array(lat) * 1e3
+
+    # batched and plain agree frame by frame
+    c = numpy.concatenate([[0], numpy.cumsum(per_frame)])
+    max_dxy = 0.0
+    flags_equal = True
+    for f in range(args.plain_frames):
+        b = out[c[f]:c[f + 1]]
+        st = status[c[f]:c[f + 1]]
+        flags_equal &= bool(numpy.array_equal(st, plain_out[f]["status"]))
+        ok = st != D.ERROR
+        if ok.any():
+            max_dxy = max(max_dxy, float(numpy.abs(b[ok, 0] - plain_out[f]["x"][ok]).max()),
+                          float(numpy.abs(b[ok, 1] - plain_out[f]["y"][ok]).max()))
+
+    log("plain: %d frames, p99 %.2f ms" % (args.plain_frames, numpy.percentile(lat_ms, 99)))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline on %d frames" % min(args.cpu_frames, F))
+        nfr = min(args.cpu_frames, F)
+        host_images_cpu = images[:nfr].cpu().numpy()
+        host_bgs_cpu = backgrounds[:nfr].cpu().numpy()
+        cpu, cpu_out = cpu_baseline(host_images_cpu, host_bgs_cpu, host_rows, counts_np, nfr)
+        # GPU batched vs the oracle on the same frames
+        max_d = 0.0
+        for f in range(min(nfr, args.plain_frames)):
+            b = out[c[f]:c[f + 1]]
+            st = status[c[f]:c[f + 1]]
+            flags_equal &= bool(numpy.array_equal(st, cpu_out[f]["status"]))
+            ok = st != D.ERROR
+            if ok.any():
+                max_d = max(max_d, float(numpy.abs(b[ok, 0] - cpu_out[f]["x"][ok]).max()),
+                            float(numpy.abs(b[ok, 1] - cpu_out[f]["y"][ok]).max()))
+        cpu["max_dxy_vs_gpu_px"] = max_d
+
+    st = bf.stats()
+    line = {
+        "metric": "converged emitter fits/sec + p99 frame latency; batched vs plain, 1 MI355X",
+        "value": batched_fps,
+        "unit": "converged fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded GPU generator: Poisson frames, offset 100 ADU, gain 1)",
+        "config": {"workload": "3D astigmatic Z model, %d frames of %dx%d, %d emitters/frame, ROI %d, MLE" %
+                               (F, S, S, N, ROI),
+                   "frames": F, "frame_size": S, "emitters_per_frame": N, "model": "Z", "roi_size": ROI},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lm (one LM turn per peak)", "lm_ms": lm_ms, "lm_launches": lm_launches,
+                     "avg_launch_us": (lm_ms * 1e3 / lm_launches) if lm_launches else None,
+                     "lm_cycles": cycles, "bytes_per_cycle": N_ROI_18 * ALG_BYTES_PER_PIXEL},
+        "cpu_baseline": cpu,
+        "plain": {"fits_per_s": plain_fps, "frames": args.plain_frames, "p50_ms": float(numpy.percentile(lat_ms, 50)),
+                  "p99_ms": float(numpy.percentile(lat_ms, 99)), "path": "MultiFitterZ, one frame per call"},
+        "batched_vs_plain": batched_fps / plain_fps if plain_fps > 0 else None,
+        "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
+        "parity": {"flags_equal_batched_plain_oracle": flags_equal, "max_dxy_batched_vs_plain_px": max_dxy},
+        "converged_per_step": conv // max(1, args.steps),
+        "scheduler": {"sweeps": st["sweeps"], "rounds": st["rounds"], "escape_reruns": st["escape_reruns"],
+                      "max_attempts": st["max_attempts"]},
+        "generate_s": t_gen,
+    }
+    print(json.dumps(line))
+    bf.cleanup()
+
+
+if __name__ == "__main__":
+    main()

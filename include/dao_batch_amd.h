@@ -1,0 +1,91 @@
+/*
+ * dao_batch_amd.h -- batch entry points of libdao_fit.so.
+ *
+ * These have no counterpart in the reference: its fitter handles one frame
+ * per fitData (storm_analysis/sa_library/dao_fit_c.py:257-549) and callers
+ * loop over frames.  A daoBatch holds many frames (of different sizes) on the
+ * GPU at once; daoBatchFit runs MultiFitter.doFit (dao_fit_c.py:332-356) for
+ * all of them together, each frame with exactly the per-frame semantics of
+ * mFitIterateLM (multi_fit.c:1072-1349): frames share no state and the result
+ * of a frame does not depend on the other frames in the batch.
+ *
+ * Frame data are concatenated in frame order, each frame row-major
+ * (height x width).  Peak rows use the daoNewPeaks layouts (n x 4 for
+ * "finder"/"testing", n x 7 for "text"/"hdf5").  Pointers flagged
+ * `on_device` are HIP device pointers (for example torch CUDA tensors).
+ * Calls are synchronous, one call per batch at a time.
+ */
+#ifndef DAO_BATCH_AMD_H
+#define DAO_BATCH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct daoBatch daoBatch;
+
+/* Model selection: shape 0 = 2DFixed, 1 = 2D, 2 = 3D, 3 = Z (dao_fit.c:1319-1571);
+ * error model 0 = MLE, 1 = ALS, 2 = LS, 3 = DWLS, 4 = FWLS (multi_fit.c:111-458;
+ * 3D and Z support MLE only, as in the reference). */
+typedef struct daoBatchModel {
+  int shape;
+  int error_model;
+  int roi_size;
+  int margin;       /* reserved-box margin of the scheduler, pixels (default 2) */
+  double tolerance; /* 1e-6 in the reference (dao_fit_c.py:286) */
+  double width_min; /* 2D / 3D width clamp, 1/(2 sigma^2) */
+  double width_max;
+  double wx[5];     /* Z: [wo (px), c (um), d (um), A, B] */
+  double wy[5];
+  double z_min;
+  double z_max;
+} daoBatchModel;
+
+/* Create a batch of n_frames frames; peak_caps[f] bounds the peaks of frame f.
+ * Returns NULL without a GPU or with unsupported sizes. */
+daoBatch *daoBatchCreate(const daoBatchModel *model, int n_frames, const int *widths, const int *heights,
+                         const int *peak_caps);
+void daoBatchDestroy(daoBatch *b);
+
+/* Per-frame camera calibration (host arrays; NULL = rqe 1 / scmos 0). */
+void daoBatchSetCalibration(daoBatch *b, int frame, const double *rqe, const double *scmos_cal);
+
+/* mFitNewImage / mFitNewBackground (multi_fit.c:1360-1408) for every frame. */
+void daoBatchNewImages(daoBatch *b, const double *images, int on_device);
+void daoBatchNewBackgrounds(daoBatch *b, const double *backgrounds, int on_device);
+/* mFitAnscombeTransformImage for every frame (ALS models). */
+void daoBatchAnscombe(daoBatch *b);
+
+/* daoNewPeaks (dao_fit.c:1582-1766) for every frame: counts[f] rows for
+ * frame f, rows concatenated in frame order. */
+void daoBatchNewPeaks(daoBatch *b, const double *rows, const int *counts, const char *p_type, int on_device);
+
+/* doFit for every frame; returns the number of sweeps run. */
+int daoBatchFit(daoBatch *b, int max_iterations);
+
+/* Per-frame read-back.  out: nfit x 9 doubles per peak
+ * {x, y, z, height, background, xsigma, ysigma, error, iterations}. */
+int daoBatchNFit(daoBatch *b, int frame);
+void daoBatchGetPeaks(daoBatch *b, int frame, double *out, int32_t *status);
+/* All frames at once, concatenated in frame order (sum of nfit rows). */
+void daoBatchGetAllPeaks(daoBatch *b, double *out, int32_t *status);
+/* Counters of the frame since the last call, fitData order:
+ * n_dposv, n_iterations, n_lost, n_margin, n_neg_fi, n_neg_height, n_non_converged, n_non_decr. */
+void daoBatchCounters(daoBatch *b, int frame, int32_t *out8);
+/* Fit image (f / rqe) of one frame. */
+void daoBatchGetFitImage(daoBatch *b, int frame, double *out);
+/* mFitRemoveErrorPeaks for every frame. */
+void daoBatchRemoveErrorPeaks(daoBatch *b);
+
+/* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
+ * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */
+void daoBatchSetTiming(daoBatch *b, int on);
+void daoBatchStats(daoBatch *b, double *out6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAO_BATCH_AMD_H */

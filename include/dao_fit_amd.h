@@ -120,6 +120,10 @@ void mFitAnscombeTransformImage(fitData *fd);
 /* mFitIterateLM (multi_fit.c:1072-1349): one sweep over the RUNNING peaks in
  * index order, then mFitRecenterPeaks (multi_fit.c:1644-1690). */
 void mFitIterateLM(fitData *fd);
+/* MultiFitter.doFit (dao_fit_c.py:332-356) in one call: no counterpart in
+ * the reference's C library (its Python loops over mFitIterateLM and
+ * mFitGetUnconverged).  Same result as that loop; returns the sweeps run. */
+int mFitDoFit(fitData *fd, int max_iterations);
 /* mFitGetUnconverged (multi_fit.c:898-910). */
 int mFitGetUnconverged(fitData *fd);
 /* mFitGetNError (multi_fit.c:730-742). */

@@ -127,4 +127,20 @@ struct OpArgs {
   int round;
 };
 
+struct FitArgs {
+  Model m;
+  Pix px;
+  Peaks pk;
+  const Frame* frames;
+  const int* roi_tab;
+  int* op_peak;         // per-slot scratch: op list of the frame
+  int4* box;            // per-slot scratch: op boxes
+  int* op_delta;        // per-slot scratch: counter increments of an op
+  int* frame_counters;  // [F][C_COUNT]
+  int* frame_sweeps;    // [F]: sweeps run by this launch
+  const int* frame_list;
+  int max_more;         // doFit's max_iterations; 0 = exactly one sweep
+  int* err_flag;        // set when a wait exceeds its (very large) bound
+};
+
 }  // namespace dao

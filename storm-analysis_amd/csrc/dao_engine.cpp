@@ -55,6 +55,8 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) fail("no HIP device available");
   DAO_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  const char* ex = std::getenv("DAO_EXECUTOR");
+  use_rounds = ex != nullptr && std::strcmp(ex, "rounds") == 0;
 
   roi_table(m.roi, roi_x, roi_y);
   m.nroi = (int)roi_x.size();
@@ -127,7 +129,7 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
 }
 
 Engine::~Engine() {
-  if (st) hipStreamSynchronize(st);
+  if (st) (void)hipStreamSynchronize(st);
   free_peaks(pk);
   free_peaks(pk2);
   free_sched();
@@ -152,8 +154,14 @@ Engine::~Engine() {
   dfree(d_roi);
   dfree(d_stage);
   dfree(d_istage);
-  if (h_pin) hipHostFree(h_pin);
-  if (st) hipStreamDestroy(st);
+  dfree(d_sweeps);
+  dfree(d_flist);
+  dfree(d_err);
+  if (h_pin) (void)hipHostFree(h_pin);
+  if (h_fr) (void)hipHostFree(h_fr);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (st) (void)hipStreamDestroy(st);
 }
 
 void Engine::set_model(const Model& mm) {
@@ -162,8 +170,14 @@ void Engine::set_model(const Model& mm) {
   m.nroi = nroi;
 }
 
+// Frame descriptors go through a pinned buffer and the copy is waited for,
+// so the host copy can change right after and earlier kernels on the stream
+// still see the old table.
 void Engine::upload_frames() {
-  DAO_CHECK(hipMemcpyAsync(d_fr, fr.data(), sizeof(Frame) * F, hipMemcpyHostToDevice, st));
+  if (h_fr == nullptr) DAO_CHECK(hipHostMalloc((void**)&h_fr, sizeof(Frame) * F, hipHostMallocDefault));
+  std::memcpy(h_fr, fr.data(), sizeof(Frame) * F);
+  DAO_CHECK(hipMemcpyAsync(d_fr, h_fr, sizeof(Frame) * F, hipMemcpyHostToDevice, st));
+  sync();
 }
 
 void Engine::alloc_peaks(Peaks& p, long long c) {
@@ -383,10 +397,23 @@ void Engine::run_ops(int kind, const double* rows, int fmt, int first_slot) {
   int round = 0;
   int group = 4;
   int last = sc.nops;
+  bool timed = timing && kind == OP_LM;
+  if (timed && ev0 == nullptr) {
+    DAO_CHECK(hipEventCreate(&ev0));
+    DAO_CHECK(hipEventCreate(&ev1));
+  }
   for (;;) {
+    if (timed) DAO_CHECK(hipEventRecord(ev0, st));
     for (int r = 0; r < group; r++) launch_op_round(kind, args(round++), st, rows, fmt, first_slot);
+    if (timed) DAO_CHECK(hipEventRecord(ev1, st));
     DAO_CHECK(hipMemcpyAsync(h, sc.remaining, sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
+    if (timed) {
+      float ms = 0.0f;
+      DAO_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+      lm_ms += ms;
+      lm_launches += group;
+    }
     int rem = h[0];
     if (rem == 0) break;
     if (rem >= last) fail("ordered executor made no progress");
@@ -404,11 +431,84 @@ int Engine::select(int mode) {
   return h[0];
 }
 
+int Engine::running_count() {
+  launch_fill_i32(sc.frame_bad, F, 0, st);
+  return select(SEL_RUNNING);
+}
+
+int Engine::fit_frames(int max_more) {
+  std::vector<int> list;
+  for (int f = 0; f < F; f++)
+    if (fr[f].nfit > 0) list.push_back(f);
+  if (list.empty()) return 0;
+  if (d_sweeps == nullptr) {
+    d_sweeps = dalloc<int>(F);
+    d_flist = dalloc<int>(F);
+    d_err = dalloc<int>(1);
+    launch_fill_i32(d_err, 1, 0, st);
+  }
+  int* h = pin(sizeof(int) * ((size_t)F + 1));
+  std::memcpy(h, list.data(), sizeof(int) * list.size());
+  DAO_CHECK(hipMemcpyAsync(d_flist, h, sizeof(int) * list.size(), hipMemcpyHostToDevice, st));
+  launch_fill_i32(d_sweeps, F, 0, st);
+  FitArgs a;
+  a.m = m;
+  a.px = px;
+  a.pk = pk;
+  a.frames = d_fr;
+  a.roi_tab = d_roi;
+  a.op_peak = sc.op_peak;
+  a.box = sc.box;
+  a.op_delta = sc.op_delta;
+  a.frame_counters = d_counters;
+  a.frame_sweeps = d_sweeps;
+  a.frame_list = d_flist;
+  a.max_more = max_more;
+  a.err_flag = d_err;
+  if (timing && ev0 == nullptr) {
+    DAO_CHECK(hipEventCreate(&ev0));
+    DAO_CHECK(hipEventCreate(&ev1));
+  }
+  if (timing) DAO_CHECK(hipEventRecord(ev0, st));
+  launch_fit_frames(a, (int)list.size(), st);
+  if (timing) DAO_CHECK(hipEventRecord(ev1, st));
+  DAO_CHECK(hipMemcpyAsync(h, d_sweeps, sizeof(int) * F, hipMemcpyDeviceToHost, st));
+  DAO_CHECK(hipMemcpyAsync(h + F, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync();
+  if (h[F] != 0) fail("a fitter wait exceeded its bound (scheduler error)");
+  int most = 0;
+  for (int f = 0; f < F; f++) most = std::max(most, h[f]);
+  sweeps_total += most;
+  if (timing) {
+    float ms = 0.0f;
+    DAO_CHECK(hipEventElapsedTime(&ms, ev0, ev1));
+    lm_ms += ms;
+    lm_launches += 1;
+  }
+  return most;
+}
+
+int Engine::fit(int max_iterations) {
+  if (!use_rounds) return fit_frames(max_iterations);
+  int sweeps = 1;
+  sweep();
+  int i = 0;
+  while (running_count() > 0 && i < max_iterations) {
+    sweep();
+    sweeps++;
+    i++;
+  }
+  return sweeps;
+}
+
 // mFitIterateLM (multi_fit.c:1072-1349) for every frame of the batch.
-void Engine::sweep() {
+int Engine::sweep() {
   launch_fill_i32((int*)d_override, cap * 4, 0, st);
   launch_fill_i32(sc.frame_bad, F, 0, st);
   sc.nops = select(SEL_RUNNING);
+  const int started = sc.nops;
+  if (started == 0) return 0;
+  sweeps_total++;
   int attempt = 0;
   while (sc.nops > 0) {
     launch_box(OP_LM, sc, pk, d_fr, m, d_override, nullptr, 0, 0, st);
@@ -438,6 +538,7 @@ void Engine::sweep() {
     run_ops(OP_RECENTER, nullptr, 0, 0);
     launch_commit(sc, pk, d_counters, st);
   }
+  return started;
 }
 
 // daoNewPeaks (dao_fit.c:1582-1766) for one frame; storage policy is the
@@ -623,5 +724,40 @@ int Engine::count_status(int f, int status) {
   for (int v : s) c += (v == status) ? 1 : 0;
   return c;
 }
+
+long long Engine::total_nfit() const {
+  long long n = 0;
+  for (int f = 0; f < F; f++) n += fr[f].nfit;
+  return n;
+}
+
+void Engine::pack_all(double* out, int32_t* status) {
+  long long total = total_nfit();
+  if (total == 0) return;
+  std::vector<int> off(F);
+  long long at = 0;
+  for (int f = 0; f < F; f++) {
+    off[f] = (int)at;
+    at += fr[f].nfit;
+  }
+  size_t rows_bytes = sizeof(double) * 9 * (size_t)total;
+  size_t stat_bytes = sizeof(int) * (size_t)total;
+  double* d = stage(rows_bytes + stat_bytes + sizeof(int) * F + 64);
+  int* d_status = (int*)((char*)d + rows_bytes);
+  int* d_off = d_status + total;
+  DAO_CHECK(hipMemcpyAsync(d_off, off.data(), sizeof(int) * F, hipMemcpyHostToDevice, st));
+  launch_pack_peaks(pk, d_fr, d_off, m, cap, d, d_status, st);
+  DAO_CHECK(hipMemcpyAsync(out, d, rows_bytes, hipMemcpyDeviceToHost, st));
+  DAO_CHECK(hipMemcpyAsync(status, d_status, stat_bytes, hipMemcpyDeviceToHost, st));
+  sync();
+}
+
+void Engine::new_images_all_dev(const double* img) {
+  launch_new_image(px, 0, npix, img, st);
+  for (int f = 0; f < F; f++) fr[f].nfit = 0;
+  upload_frames();
+}
+
+void Engine::new_backgrounds_all_dev(const double* bg) { launch_new_background(px, 0, npix, bg, st); }
 
 }  // namespace dao

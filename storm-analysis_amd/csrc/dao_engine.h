@@ -42,6 +42,9 @@ void launch_select(const Peaks& pk, const Frame* frames, long long nslots, int m
                    int* flag, int* out, int* count, hipStream_t st);
 void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
 void launch_iota(int* p, int n, int base, hipStream_t st);
+void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st);
+void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, const Model& m, long long nslots,
+                       double* out, int* status, hipStream_t st);
 
 // Reference ROI table (multi_fit.c:1031-1060): pixels whose centre lies within
 // r/2 of the ROI centre, row-major.
@@ -64,6 +67,7 @@ class Engine {
   int F = 0;
   std::vector<Frame> fr;
   Frame* d_fr = nullptr;
+  Frame* h_fr = nullptr;  // pinned copy of fr for uploads
   Pix px{};
   long long npix = 0;
   Peaks pk{}, pk2{};
@@ -103,7 +107,23 @@ class Engine {
   void compact(int f, const std::vector<int32_t>& keep);
   void set_status(int f, const int32_t* status);
   // fitting
-  void sweep();  // one mFitIterateLM over every frame of the batch
+  // one mFitIterateLM over every frame of the batch; returns the number of
+  // RUNNING peaks it started with (0: nothing to do)
+  int sweep();
+  // MultiFitter.doFit (dao_fit_c.py:332-356) for every frame: one sweep, then
+  // more while any peak is RUNNING, at most max_iterations more; returns the
+  // number of sweeps
+  int fit(int max_iterations);
+  int running_count();
+  // sweeps of every frame with peaks, one persistent workgroup per frame:
+  // max_more = 0 runs exactly one sweep (mFitIterateLM), otherwise doFit
+  int fit_frames(int max_more);
+  bool use_rounds = false;  // multi-kernel executor instead of fit_frames
+  // timing of the LM kernel (HIP events on the engine's stream)
+  bool timing = false;
+  double lm_ms = 0.0;
+  long long lm_launches = 0;
+  long long sweeps_total = 0;
   // read-back
   void read_counters(std::vector<int>& out);  // F * C_COUNT, zeroes the device copy
   void get_status(int f, std::vector<int32_t>& out);
@@ -113,6 +133,12 @@ class Engine {
   void get_sums(int f, int what, std::vector<double>& out);
   void get_jacobian(int f, std::vector<double>& out);
   void get_image(int f, double* out_host, int residual);
+  // all frames' live peaks as rows of 9 doubles + status (host buffers)
+  void pack_all(double* out_host, int32_t* status_host);
+  long long total_nfit() const;
+  // every frame at once from concatenated device arrays
+  void new_images_all_dev(const double* img_dev);
+  void new_backgrounds_all_dev(const double* bg_dev);
   int count_status(int f, int status);
   void sync();
 
@@ -127,6 +153,10 @@ class Engine {
   int* istage(size_t bytes);
   int select(int mode);
   void run_ops(int kind, const double* rows, int fmt, int first_slot);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int* d_sweeps = nullptr;
+  int* d_flist = nullptr;
+  int* d_err = nullptr;
   OpArgs args(int round);
 };
 

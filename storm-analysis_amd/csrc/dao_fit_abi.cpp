@@ -201,8 +201,26 @@ void mFitAnscombeTransformImage(fitData* fd) { H(fd)->eng->anscombe(0); }
 
 void mFitIterateLM(fitData* fd) {
   Handle* h = H(fd);
-  h->eng->sweep();
+  if (h->eng->use_rounds)
+    h->eng->sweep();
+  else
+    h->eng->fit_frames(0);
   add_counters(h);
+}
+
+// doFit in one call: one sweep, then more while any peak is RUNNING, at most
+// max_iterations more (dao_fit_c.py:332-356); the same result as calling
+// mFitIterateLM / mFitGetUnconverged in that loop.  Returns the sweeps run.
+int mFitDoFit(fitData* fd, int max_iterations) {
+  Handle* h = H(fd);
+  int s;
+  if (h->eng->use_rounds) {
+    s = h->eng->fit(max_iterations);
+  } else {
+    s = h->eng->fit_frames(max_iterations);
+  }
+  add_counters(h);
+  return s;
 }
 
 int mFitGetUnconverged(fitData* fd) { return H(fd)->eng->count_status(0, dao::ST_RUNNING); }

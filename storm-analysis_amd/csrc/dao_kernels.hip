@@ -142,40 +142,57 @@ __device__ inline void lane_shape(const Model& m, const PeakState& s, double& t,
 
 template <int PPL>
 struct Roi {
-  int rx[PPL], ry[PPL];
-  bool ok[PPL];
-  double f[PPL], bg[PPL], x[PPL], rqe[PPL], scm[PPL], tfi[PPL], psf[PPL];
+  // per slot: column | row << 8 of the ROI pixel, and its offset row*W+col
+  int rxy[PPL];
+  int koff[PPL];
+  double f[PPL], bg[PPL], tfi[PPL], psf[PPL];
   int cnt[PPL];
-  uint8_t stale[PPL];
-  int at_x, at_y;  // ROI origin the cache holds
+  unsigned stale;  // bit s: slot s is stale
+  unsigned okm;    // bit s: slot s is a pixel of the ROI
+  long long base;  // pixel index of the ROI origin the cache holds
+  int at_x, at_y;
   bool valid;
+  // read-only pixel data, read from memory when needed
+  const double* x;
+  const double* rqe;
+  const double* scm;
 
-  __device__ void init(const Model& m, const int* roi_tab) {
+  __device__ bool ok(int s) const { return (okm >> s) & 1u; }
+  __device__ int rx(int s) const { return rxy[s] & 0xff; }
+  __device__ int ry(int s) const { return rxy[s] >> 8; }
+  __device__ long long k(int s) const { return base + koff[s]; }
+
+  __device__ void init(const Model& m, const int* roi_tab, const Pix& px, const Frame& fr) {
     int l = lane_id();
+    okm = 0;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
       int j = l + s * kWave;
-      ok[s] = j < m.nroi;
-      int v = ok[s] ? roi_tab[j] : 0;
-      rx[s] = v & 0xffff;
-      ry[s] = v >> 16;
+      bool o = j < m.nroi;
+      int v = o ? roi_tab[j] : 0;
+      int cx = v & 0xffff, cy = v >> 16;
+      rxy[s] = cx | (cy << 8);
+      koff[s] = cy * fr.W + cx;
+      if (o) okm |= 1u << s;
     }
+    x = px.x;
+    rqe = px.rqe;
+    scm = px.scmos;
     valid = false;
   }
 
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
+    base = fr.pix0 + (long long)yi * fr.W + xi;
+    stale = 0;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      if (ok[s]) {
-        long long k = fr.pix0 + (long long)(yi + ry[s]) * fr.W + (xi + rx[s]);
-        f[s] = px.f[k];
-        bg[s] = px.bg[k];
-        cnt[s] = px.cnt[k];
-        x[s] = px.x[k];
-        rqe[s] = px.rqe[k];
-        scm[s] = px.scmos[k];
-        tfi[s] = px.tfi[k];
-        stale[s] = px.stale[k];
+      if (ok(s)) {
+        long long q = k(s);
+        f[s] = px.f[q];
+        bg[s] = px.bg[q];
+        cnt[s] = px.cnt[q];
+        tfi[s] = px.tfi[q];
+        if (px.stale[q]) stale |= 1u << s;
       }
     }
     at_x = xi;
@@ -187,13 +204,13 @@ struct Roi {
     if (!valid) return;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      if (ok[s]) {
-        long long k = fr.pix0 + (long long)(at_y + ry[s]) * fr.W + (at_x + rx[s]);
-        px.f[k] = f[s];
-        px.bg[k] = bg[s];
-        px.cnt[k] = cnt[s];
-        px.tfi[k] = tfi[s];
-        px.stale[k] = stale[s];
+      if (ok(s)) {
+        long long q = k(s);
+        px.f[q] = f[s];
+        px.bg[q] = bg[s];
+        px.cnt[q] = cnt[s];
+        px.tfi[q] = tfi[s];
+        px.stale[q] = (stale >> s) & 1u;
       }
     }
   }
@@ -205,14 +222,14 @@ struct Roi {
     load(px, fr, xi, yi);
   }
 
-  // psf at the cache's pixels for state s (cache must be at s.xi, s.yi).
+  // psf at the cache's pixels for state st (cache must be at st.xi, st.yi).
   __device__ void shape(const Model& m, const PeakState& st) {
     double t, e;
     lane_shape(m, st, t, e);
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      double ex = __shfl(e, rx[s], kWave);
-      double ey = __shfl(e, 32 + ry[s], kWave);
+      double ex = __shfl(e, rx(s), kWave);
+      double ey = __shfl(e, 32 + ry(s), kWave);
       psf[s] = ey * ex;
     }
   }
@@ -223,19 +240,22 @@ struct Roi {
     double b = st.prm[P_BG];
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      if (ok[s]) {
+      if (ok(s)) {
+        long long q = k(s);
+        double r = rqe[q];
+        double sc = scm[q];
         if (sign > 0) {
-          f[s] += h * psf[s] * rqe[s];
+          f[s] += h * psf[s] * r;
           cnt[s] += 1;
-          bg[s] += b * rqe[s] + scm[s];
+          bg[s] += b * r + sc;
         } else {
-          f[s] -= h * psf[s] * rqe[s];
+          f[s] -= h * psf[s] * r;
           cnt[s] -= 1;
-          bg[s] -= (b * rqe[s] + scm[s]);
+          bg[s] -= (b * r + sc);
         }
-        stale[s] = 1;
       }
     }
+    stale = okm;
   }
 };
 
@@ -257,7 +277,7 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx_frame, 
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
     fi[s] = 0.0;
-    if (c.ok[s]) {
+    if (c.ok(s)) {
       fi[s] = c.f[s] + c.bg[s] / ((double)c.cnt[s]);
       if ((m.errm == EM_MLE || m.errm == EM_FWLS) && fi[s] <= 0.0) {
         int j = l + s * kWave;
@@ -271,18 +291,18 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx_frame, 
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
     int j = l + s * kWave;
-    if (c.ok[s] && j < first_bad) {
-      double xk = c.x[s];
+    if (c.ok(s) && j < first_bad) {
+      long long q = c.k(s);
+      double xk = c.x[q];
       double e;
       double t = fi[s];
       if (m.errm == EM_MLE) {
         e = 2.0 * ((t - xk) - xk * log(t / xk));
       } else if (m.errm == EM_ALS) {
         double a = anscombe(t);
-        if (a == 0.0 && c.stale[s]) zeros++;
+        if (a == 0.0 && ((c.stale >> s) & 1u)) zeros++;
         t = a;
-        long long k = fr.pix0 + (long long)(c.at_y + c.ry[s]) * fr.W + (c.at_x + c.rx[s]);
-        double d = a - asx_frame[k];
+        double d = a - asx_frame[q];
         e = d * d;
       } else if (m.errm == EM_LS) {
         double d = t - xk;
@@ -295,7 +315,7 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx_frame, 
         e = d * d / t;
       }
       c.tfi[s] = t;
-      c.stale[s] = 0;
+      c.stale &= ~(1u << s);
       sum += e;
     }
   }
@@ -332,13 +352,14 @@ __device__ void calc_jh(const Model& m, const Roi<PPL>& c, const PeakState& st, 
 
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
-    double xt = __shfl(t, c.rx[s], kWave);
-    double ex = __shfl(e, c.rx[s], kWave);
-    double yt = __shfl(t, 32 + c.ry[s], kWave);
-    double ey = __shfl(e, 32 + c.ry[s], kWave);
-    if (c.ok[s]) {
+    double xt = __shfl(t, c.rx(s), kWave);
+    double ex = __shfl(e, c.rx(s), kWave);
+    double yt = __shfl(t, 32 + c.ry(s), kWave);
+    double ey = __shfl(e, 32 + c.ry(s), kWave);
+    if (c.ok(s)) {
+      const long long kq = c.k(s);
       double fi = c.tfi[s];
-      double xk = c.x[s];
+      double xk = c.x[kq];
       double et = ex * ey;
       double g[N];
       if (SHAPE == SH_2DFIXED) {
@@ -366,7 +387,7 @@ __device__ void calc_jh(const Model& m, const Roi<PPL>& c, const PeakState& st, 
         g[3] = -a1 * xt * xt * gx * et - a1 * yt * yt * gy * et;
         g[4] = 1.0;
       }
-      double sc = (m.errm == EM_ALS) ? c.rqe[s] * 2.0 / fi : c.rqe[s];
+      double sc = (m.errm == EM_ALS) ? c.rqe[kq] * 2.0 / fi : c.rqe[kq];
 #pragma unroll
       for (int a = 0; a < N; a++) g[a] *= sc;
       double wj, wh;
@@ -374,8 +395,7 @@ __device__ void calc_jh(const Model& m, const Roi<PPL>& c, const PeakState& st, 
         wj = 2.0 * (1.0 - xk / fi);
         wh = 2.0 * xk / (fi * fi);
       } else if (m.errm == EM_ALS) {
-        long long k = fr.pix0 + (long long)(c.at_y + c.ry[s]) * fr.W + (c.at_x + c.rx[s]);
-        wj = fi - asx[k];
+        wj = fi - asx[kq];
         wh = 1.0;
       } else if (m.errm == EM_LS) {
         wj = fi - xk;
@@ -536,12 +556,12 @@ __device__ inline int peak_check(const Model& m, const Frame& fr, PeakState& s, 
   return 0;
 }
 
-// mFitResetPeak (multi_fit.c:1788-1798): back to the turn's start, keeping
-// `added`, lambda times 4.
-__device__ inline void reset_to(PeakState& w, const PeakState& s0) {
+// mFitResetPeak (multi_fit.c:1788-1798): back to the turn's start (still in
+// the peak's arrays), keeping `added`, lambda times 4.
+__device__ inline void reset_from(PeakState& w, const Peaks& pk, int p) {
   int added = w.added;
   double lam = w.lam;
-  w = s0;
+  load_state(pk, p, w);
   w.added = added;
   w.lam = lam * 4.0;
 }
@@ -639,7 +659,8 @@ __global__ void k_bin_scatter(Sched sc, const Frame* frames, const Peaks pk) {
   const Frame fr = frames[pk.frame[sc.op_peak[i]]];
   int c = cell_of(sc, fr, b);
   int q = atomicAdd(&sc.cell_fill[c], 1);
-  sc.cell_ops[sc.cell_start[c] + q] = i;
+  int pos = sc.cell_start[c] + q;
+  if (pos < sc.nops) sc.cell_ops[pos] = i;
 }
 
 // Predecessors of op i: earlier ops of the same frame with overlapping boxes.
@@ -726,44 +747,36 @@ __device__ inline void op_finish(const Sched& sc, int i, int round, int flags, c
 // op kernels
 // ---------------------------------------------------------------------------
 
-// One Levenberg-Marquardt turn of one peak: the body of the loop at
-// multi_fit.c:1089-1345.
+// One Levenberg-Marquardt turn of peak p: the body of the loop at
+// multi_fit.c:1089-1345.  The turn may touch only pixels inside `box`.
+// Returns 1 when the turn completed, 2 when a candidate ROI left the box: the
+// frame is then exactly as before the turn (the starting contribution is put
+// back), the peak state is untouched, and esc_box holds the box needed.
+// d[] receives the turn's counter increments.
 template <int PPL, int SHAPE>
-__global__ __launch_bounds__(256) void k_lm(OpArgs a) {
+__device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
+                       int p, int4 box, int* d, int4& esc_box) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
-  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (i >= a.sc.nops) return;
-  if (a.sc.done[i] >= 0) return;
-  if (!op_ready(a.sc, a.frames, a.pk, i, a.round)) return;
-
-  const Model& m = a.m;
-  const int p = a.sc.op_peak[i];
-  const Frame fr = a.frames[a.pk.frame[p]];
-  const int4 box = a.sc.box[i];
-  int d[C_COUNT];
-#pragma unroll
-  for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-
-  PeakState s0, w;
-  load_state(a.pk, p, s0);
-  w = s0;
+  // The peak's arrays keep the turn's starting state until the turn ends, so
+  // resets read it back from there instead of holding a second copy.
+  PeakState w;
+  load_state(pk, p, w);
   if (!origin_ok(m, fr, w.xi, w.yi)) {
     w.status = ST_ERROR;
-    store_state(a.pk, p, w);
-    op_finish(a.sc, i, a.round, 1, d);
-    return;
+    store_state(pk, p, w);
+    return 1;
   }
 
   Roi<PPL> c;
-  c.init(m, a.roi_tab);
-  c.load(a.px, fr, w.xi, w.yi);
+  c.init(m, roi_tab, px, fr);
+  c.load(px, fr, w.xi, w.yi);
   c.shape(m, w);
 
   double J[N], H[N * (N + 1) / 2], wJ[N];
   double e;
-  if (eval_error<PPL>(m, c, a.px.asx, fr, e, d[C_NEG_FI]) == 0) w.err = e;
+  if (eval_error<PPL>(m, c, px.asx, fr, e, d[C_NEG_FI]) == 0) w.err = e;
   const double start = w.err;
-  calc_jh<PPL, SHAPE>(m, c, w, a.px.asx, fr, J, H);
+  calc_jh<PPL, SHAPE>(m, c, w, px.asx, fr, J, H);
   c.stamp(w, -1);
   w.added--;
 
@@ -791,7 +804,7 @@ __global__ __launch_bounds__(256) void k_lm(OpArgs a) {
 
     apply_update<SHAPE>(m, w, wJ);
     if (peak_check(m, fr, w, d)) {
-      reset_to(w, s0);
+      reset_from(w, pk, p);
       w.status = ST_ERROR;
       continue;
     }
@@ -801,15 +814,15 @@ __global__ __launch_bounds__(256) void k_lm(OpArgs a) {
       break;
     }
 
-    c.move_to(a.px, fr, w.xi, w.yi);
+    c.move_to(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
     w.added++;
 
-    if (eval_error<PPL>(m, c, a.px.asx, fr, e, d[C_NEG_FI]) != 0) {
+    if (eval_error<PPL>(m, c, px.asx, fr, e, d[C_NEG_FI]) != 0) {
       c.stamp(w, -1);
       w.added--;
-      reset_to(w, s0);
+      reset_from(w, pk, p);
       w.status = ST_ERROR;
       continue;
     }
@@ -823,7 +836,7 @@ __global__ __launch_bounds__(256) void k_lm(OpArgs a) {
       d[C_NON_DECR]++;
       c.stamp(w, -1);
       w.added--;
-      reset_to(w, s0);
+      reset_from(w, pk, p);
     } else {
       if (((start - w.err) / start) < m.tol) {
         w.status = ST_CONVERGED;
@@ -837,55 +850,43 @@ __global__ __launch_bounds__(256) void k_lm(OpArgs a) {
   if (escaped) {
     // Put the frame back: re-add the turn's starting contribution.  The
     // escape is detected before the candidate touches any pixel.
-    c.move_to(a.px, fr, s0.xi, s0.yi);
-    c.shape(m, s0);
-    c.stamp(s0, +1);
-    c.store(a.px, fr);
-    if (lane_id() == 0) {
-      int4 nb;
-      nb.x = min(box.x, w.xi - m.margin);
-      nb.y = min(box.y, w.yi - m.margin);
-      nb.z = max(box.z, w.xi + m.roi + m.margin);
-      nb.w = max(box.w, w.yi + m.roi + m.margin);
-      a.sc.esc_box[i] = nb;
-      a.sc.frame_bad[a.pk.frame[p]] = 1;
-    }
+    const int cx = w.xi, cy = w.yi;
+    load_state(pk, p, w);
+    c.move_to(px, fr, w.xi, w.yi);
+    c.shape(m, w);
+    c.stamp(w, +1);
+    c.store(px, fr);
+    w.xi = cx;
+    w.yi = cy;
+    esc_box.x = min(box.x, w.xi - m.margin);
+    esc_box.y = min(box.y, w.yi - m.margin);
+    esc_box.z = max(box.z, w.xi + m.roi + m.margin);
+    esc_box.w = max(box.w, w.yi + m.roi + m.margin);
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-    op_finish(a.sc, i, a.round, 2, d);
-    return;
+    return 2;
   }
 
-  c.store(a.px, fr);
-  store_state(a.pk, p, w);
-  op_finish(a.sc, i, a.round, 1, d);
+  c.store(px, fr);
+  store_state(pk, p, w);
+  return 1;
 }
 
-// mFitRecenterPeaks body (multi_fit.c:1656-1688) for one peak whose centre
+// mFitRecenterPeaks body (multi_fit.c:1656-1688) for peak p, whose centre
 // left [-0.25, 1.25] of its ROI origin.
 template <int PPL>
-__global__ __launch_bounds__(256) void k_recenter(OpArgs a) {
-  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (i >= a.sc.nops) return;
-  if (a.sc.done[i] >= 0) return;
-  if (!op_ready(a.sc, a.frames, a.pk, i, a.round)) return;
-  const Model& m = a.m;
-  const int p = a.sc.op_peak[i];
-  const Frame fr = a.frames[a.pk.frame[p]];
-  int d[C_COUNT];
-#pragma unroll
-  for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+__device__ void recenter_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr,
+                              const int* roi_tab, int p, int* d) {
   PeakState w;
-  load_state(a.pk, p, w);
+  load_state(pk, p, w);
   if (!origin_ok(m, fr, w.xi, w.yi)) {
     w.status = ST_ERROR;
-    store_state(a.pk, p, w);
-    op_finish(a.sc, i, a.round, 1, d);
+    store_state(pk, p, w);
     return;
   }
   Roi<PPL> c;
-  c.init(m, a.roi_tab);
-  c.load(a.px, fr, w.xi, w.yi);
+  c.init(m, roi_tab, px, fr);
+  c.load(px, fr, w.xi, w.yi);
   c.shape(m, w);
   c.stamp(w, -1);
   w.added--;
@@ -895,13 +896,100 @@ __global__ __launch_bounds__(256) void k_recenter(OpArgs a) {
     d[C_LOST] += 1;
     w.status = ST_ERROR;
   } else {
-    c.move_to(a.px, fr, w.xi, w.yi);
+    c.move_to(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
     w.added++;
   }
-  c.store(a.px, fr);
-  store_state(a.pk, p, w);
+  c.store(px, fr);
+  store_state(pk, p, w);
+}
+
+// Add (sign +1) or remove (-1) `added` copies of state s's contribution with
+// atomic adds.
+template <int PPL>
+__device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const Roi<PPL>& c,
+                           const PeakState& s, int sign) {
+  const int n = s.added * sign;
+  if (n == 0 || !origin_ok(m, fr, s.xi, s.yi)) return;
+  double t, e;
+  lane_shape(m, s, t, e);
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    double ex = __shfl(e, c.rx(q), kWave);
+    double ey = __shfl(e, 32 + c.ry(q), kWave);
+    if (c.ok(q)) {
+      long long k = fr.pix0 + (long long)s.yi * fr.W + s.xi + c.koff[q];
+      double r = px.rqe[k];
+      atomicAdd(&px.f[k], (double)n * (s.prm[P_HEIGHT] * (ey * ex) * r));
+      atomicAdd(&px.bg[k], (double)n * (s.prm[P_BG] * r + px.scmos[k]));
+      atomicAdd(&px.cnt[k], n);
+      px.stale[k] = 1;
+    }
+  }
+}
+
+// Undo a completed LM turn of peak p: the image loses the peak's current
+// contribution and regains the one it had at the attempt's start (backup),
+// and the peak state returns to the backup.  Callers order or serialise
+// undos of overlapping peaks; atomic adds keep concurrent undos safe.
+template <int PPL>
+__device__ void undo_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
+                          int p) {
+  PeakState w, s0;
+  load_state(pk, p, w);
+  s0 = w;
+#pragma unroll
+  for (int q = 0; q < kNFit; q++) s0.prm[q] = pk.bk_prm[(long long)q * pk.cap + p];
+  s0.lam = pk.bk_lam[p];
+  s0.err = pk.bk_err[p];
+  s0.xi = pk.bk_xi[p];
+  s0.yi = pk.bk_yi[p];
+  s0.status = pk.bk_status[p];
+  s0.iter = pk.bk_iter[p];
+  s0.added = pk.bk_added[p];
+  Roi<PPL> c;
+  c.init(m, roi_tab, px, fr);
+  undo_stamp<PPL>(m, px, fr, c, w, -1);
+  undo_stamp<PPL>(m, px, fr, c, s0, +1);
+  store_state(pk, p, s0);
+}
+
+// Executor round of LM turns (multi-kernel path).
+template <int PPL, int SHAPE>
+__global__ __launch_bounds__(256) void k_lm(OpArgs a) {
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= a.sc.nops) return;
+  if (a.sc.done[i] >= 0) return;
+  if (!op_ready(a.sc, a.frames, a.pk, i, a.round)) return;
+  const int p = a.sc.op_peak[i];
+  const int fi = a.pk.frame[p];
+  const Frame fr = a.frames[fi];
+  int d[C_COUNT];
+#pragma unroll
+  for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+  int4 eb;
+  int r = lm_turn<PPL, SHAPE>(a.m, a.px, a.pk, fr, a.roi_tab, p, a.sc.box[i], d, eb);
+  if (r == 2 && lane_id() == 0) {
+    a.sc.esc_box[i] = eb;
+    a.sc.frame_bad[fi] = 1;
+  }
+  op_finish(a.sc, i, a.round, r, d);
+}
+
+// Executor round of recentering (multi-kernel path).
+template <int PPL>
+__global__ __launch_bounds__(256) void k_recenter(OpArgs a) {
+  const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (i >= a.sc.nops) return;
+  if (a.sc.done[i] >= 0) return;
+  if (!op_ready(a.sc, a.frames, a.pk, i, a.round)) return;
+  const int p = a.sc.op_peak[i];
+  const Frame fr = a.frames[a.pk.frame[p]];
+  int d[C_COUNT];
+#pragma unroll
+  for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+  recenter_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, p, d);
   op_finish(a.sc, i, a.round, 1, d);
 }
 
@@ -962,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_newpeak(OpArgs a, const double* rows, i
 
   bool good = peak_check(m, fr, w, d) == 0;
   Roi<PPL> c;
-  c.init(m, a.roi_tab);
+  c.init(m, a.roi_tab, a.px, fr);
   if (good) {
     c.load(a.px, fr, w.xi, w.yi);
     c.shape(m, w);
@@ -971,9 +1059,9 @@ __global__ __launch_bounds__(256) void k_newpeak(OpArgs a, const double* rows, i
       double sb = 0.0;
 #pragma unroll
       for (int s = 0; s < PPL; s++) {
-        if (c.ok[s]) {
-          long long k = fr.pix0 + (long long)(w.yi + c.ry[s]) * fr.W + (w.xi + c.rx[s]);
-          sb += a.px.bgest[k] / c.rqe[s];
+        if (c.ok(s)) {
+          long long k = c.k(s);
+          sb += a.px.bgest[k] / c.rqe[k];
         }
       }
       w.prm[P_BG] = wsum(sb) / ((double)m.nroi);
@@ -982,11 +1070,11 @@ __global__ __launch_bounds__(256) void k_newpeak(OpArgs a, const double* rows, i
         double sp = 0.0, sx = 0.0;
 #pragma unroll
         for (int s = 0; s < PPL; s++) {
-          if (c.ok[s]) {
-            long long k = fr.pix0 + (long long)(w.yi + c.ry[s]) * fr.W + (w.xi + c.rx[s]);
-            double t = c.psf[s] * c.rqe[s];
+          if (c.ok(s)) {
+            long long k = c.k(s);
+            double t = c.psf[s] * c.rqe[k];
             sp += t * t * t;
-            sx += t * t * (c.x[s] - c.f[s] - a.px.bgest[k]);
+            sx += t * t * (c.x[k] - c.f[s] - a.px.bgest[k]);
           }
         }
         w.prm[P_HEIGHT] = wsum(sx) / wsum(sp);
@@ -1026,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_peakerror(OpArgs a) {
   load_state(a.pk, p, w);
   if (w.status == ST_RUNNING && origin_ok(m, fr, w.xi, w.yi)) {
     Roi<PPL> c;
-    c.init(m, a.roi_tab);
+    c.init(m, a.roi_tab, a.px, fr);
     c.load(a.px, fr, w.xi, w.yi);
     double e;
     if (eval_error<PPL>(m, c, a.px.asx, fr, e, d[C_NEG_FI]) == 0) w.err = e;
@@ -1053,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_subtract(OpArgs a) {
   load_state(a.pk, p, w);
   if (origin_ok(m, fr, w.xi, w.yi)) {
     Roi<PPL> c;
-    c.init(m, a.roi_tab);
+    c.init(m, a.roi_tab, a.px, fr);
     c.load(a.px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, -1);
@@ -1064,9 +1152,8 @@ __global__ __launch_bounds__(256) void k_subtract(OpArgs a) {
   op_finish(a.sc, i, a.round, 1, d);
 }
 
-// Undo the completed LM ops of frames flagged bad: the image loses the op's
-// new contribution and regains the old one (atomics: ops of one frame may
-// overlap here), and the peak state returns to the attempt's start.
+// Undo the completed LM ops of frames flagged bad (multi-kernel path; the
+// ops of one frame may overlap, the undo uses atomic adds).
 template <int PPL>
 __global__ __launch_bounds__(256) void k_rollback(OpArgs a) {
   const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
@@ -1074,44 +1161,235 @@ __global__ __launch_bounds__(256) void k_rollback(OpArgs a) {
   const int p = a.sc.op_peak[i];
   const int fi = a.pk.frame[p];
   if (!a.sc.frame_bad[fi] || (a.sc.op_flags[i] & 1) == 0) return;
-  const Model& m = a.m;
-  const Frame fr = a.frames[fi];
-  PeakState w, s0;
-  load_state(a.pk, p, w);
-  s0 = w;
-#pragma unroll
-  for (int q = 0; q < kNFit; q++) s0.prm[q] = a.pk.bk_prm[(long long)q * a.pk.cap + p];
-  s0.lam = a.pk.bk_lam[p];
-  s0.err = a.pk.bk_err[p];
-  s0.xi = a.pk.bk_xi[p];
-  s0.yi = a.pk.bk_yi[p];
-  s0.status = a.pk.bk_status[p];
-  s0.iter = a.pk.bk_iter[p];
-  s0.added = a.pk.bk_added[p];
-  Roi<PPL> c;
-  c.init(m, a.roi_tab);
-  for (int pass = 0; pass < 2; pass++) {
-    const PeakState& s = pass == 0 ? w : s0;
-    int n = s.added;
-    if (n == 0 || !origin_ok(m, fr, s.xi, s.yi)) continue;
-    double t, e;
-    lane_shape(m, s, t, e);
-    double sign = pass == 0 ? -(double)n : (double)n;
-#pragma unroll
-    for (int q = 0; q < PPL; q++) {
-      double ex = __shfl(e, c.rx[q], kWave);
-      double ey = __shfl(e, 32 + c.ry[q], kWave);
-      if (c.ok[q]) {
-        long long k = fr.pix0 + (long long)(s.yi + c.ry[q]) * fr.W + (s.xi + c.rx[q]);
-        double r = a.px.rqe[k];
-        atomicAdd(&a.px.f[k], sign * (s.prm[P_HEIGHT] * (ey * ex) * r));
-        atomicAdd(&a.px.bg[k], sign * (s.prm[P_BG] * r + a.px.scmos[k]));
-        atomicAdd(&a.px.cnt[k], pass == 0 ? -n : n);
-        a.px.stale[k] = 1;
+  undo_turn<PPL>(a.m, a.px, a.pk, a.frames[fi], a.roi_tab, p);
+}
+
+// ---------------------------------------------------------------------------
+// Frame-per-workgroup fitter
+// ---------------------------------------------------------------------------
+//
+// One workgroup of NW waves runs whole sweeps of one frame (doFit,
+// dao_fit_c.py:332-356, or one mFitIterateLM) without returning to the host.
+// The running peaks of a sweep are the ops 0..n-1 in index order; wave v
+// takes ops v, v+NW, v+2NW, ... in order and publishes in fin[v] the index
+// of its next unfinished op.  Op j starts when no unfinished op i < j has a
+// box overlapping op j's box (ops i < j with i >= fin[i % NW]).  The wave
+// holding the smallest unfinished op can always start it, so the frame makes
+// progress; overlapping ops run in index order, which is the reference's
+// update order (multi_fit.c:1089).  An LM turn whose ROI would leave its box
+// stops the others: the ops after it that already finished are undone in
+// reverse order, and the sweep resumes at the turn, whose box is now the
+// whole frame (so it runs alone and cannot escape again).
+
+constexpr int kNoEscape = 0x7fffffff;
+
+template <int NW>
+struct FpwShared {
+  int esc;
+  int fin[NW];
+  int wsum[NW];
+  int cnt[C_COUNT];
+};
+
+// Order-preserving list of the frame's peaks that satisfy `mode`
+// (SEL_RUNNING or SEL_RECENTER) into op_peak[peak0 + k]; returns the count.
+template <int NW>
+__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<NW>& sh) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  int count = 0;  // every thread keeps the same running total
+  for (int base = 0; base < fr.nfit; base += NW * kWave) {
+    int t = base + threadIdx.x;
+    bool flag = false;
+    if (t < fr.nfit) {
+      int p = fr.peak0 + t;
+      if (a.pk.status[p] == ST_RUNNING) {
+        if (mode == SEL_RUNNING) {
+          flag = true;
+        } else {
+          double dx = a.pk.prm[(long long)P_XC * a.pk.cap + p] - a.pk.xi[p];
+          double dy = a.pk.prm[(long long)P_YC * a.pk.cap + p] - a.pk.yi[p];
+          flag = (dx < -0.25) || (dx > 1.25) || (dy < -0.25) || (dy > 1.25);
+        }
       }
     }
+    unsigned long long mask = __ballot(flag);
+    int pre = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) sh.wsum[wv] = __popcll(mask);
+    __syncthreads();
+    int off = count;
+    int tot = 0;
+    for (int u = 0; u < NW; u++) {
+      if (u < wv) off += sh.wsum[u];
+      tot += sh.wsum[u];
+    }
+    if (flag) a.op_peak[fr.peak0 + off + pre] = fr.peak0 + t;
+    count += tot;
+    __syncthreads();
   }
-  store_state(a.pk, p, s0);
+  return count;
+}
+
+// Wait until op j may start.  Returns false when an earlier op escaped (the
+// op must not run now) or the wait bound was hit.
+template <int NW>
+__device__ bool fpw_wait(const FitArgs& a, const Frame& fr, int j, int4 b, FpwShared<NW>& sh) {
+  const int lane = lane_id();
+  for (long long spin = 0;; spin++) {
+    bool blocked = false;
+    if (lane < NW) {
+      int u = lane;
+      int k = __hip_atomic_load(&sh.fin[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      for (; k < j; k += NW) {
+        if (boxes_overlap(b, a.box[fr.peak0 + k])) {
+          blocked = true;
+          break;
+        }
+      }
+    }
+    if (!__any(blocked)) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      return true;
+    }
+    if (__hip_atomic_load(&sh.esc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < j) return false;
+    if (spin > (1ll << 22)) {
+      if (lane == 0) atomicExch(a.err_flag, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Run ops 0..n-1 (kind OP_LM or OP_RECENTER) of the frame.
+template <int PPL, int SHAPE, int NW>
+__device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwShared<NW>& sh) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const int4 whole = make_int4(0, 0, fr.W, fr.H);
+  int from = 0;  // ops < from are finished
+  for (;;) {
+    if (threadIdx.x < NW) {
+      int v = threadIdx.x;
+      int first = from + ((v - from % NW) % NW + NW) % NW;  // smallest index >= from owned by v
+      sh.fin[v] = first;
+    }
+    if (threadIdx.x == 0) sh.esc = kNoEscape;
+    __syncthreads();
+    int first = from + ((wv - from % NW) % NW + NW) % NW;
+    for (int j = first; j < n; j += NW) {
+      if (__hip_atomic_load(&sh.esc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < j) break;
+      const int slot = fr.peak0 + j;
+      const int4 b = a.box[slot];
+      // the redone op runs first and alone: the whole frame is its box
+      if (!fpw_wait<NW>(a, fr, j, b, sh)) break;
+      const int p = a.op_peak[slot];
+      int d[C_COUNT];
+#pragma unroll
+      for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+      int r = 1;
+      if (kind == OP_LM) {
+        int4 eb;
+        r = lm_turn<PPL, SHAPE>(a.m, a.px, a.pk, fr, a.roi_tab, p, b, d, eb);
+      } else {
+        recenter_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, p, d);
+      }
+      if (r == 2) {
+        if (lane == 0) atomicMin(&sh.esc, j);
+        break;
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < C_COUNT; q++) {
+          a.op_delta[(long long)slot * C_COUNT + q] = d[q];
+          if (d[q]) atomicAdd(&sh.cnt[q], d[q]);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) __hip_atomic_store(&sh.fin[wv], j + NW, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    const int e = sh.esc;
+    if (e == kNoEscape) break;
+    // An LM turn left its box.  Every op < e has finished; undo the finished
+    // ops > e (newest first), redo op e alone with the whole frame as its
+    // box, and continue after it.
+    if (wv == 0) {
+      int hi = 0;
+      for (int u = 0; u < NW; u++) hi = max(hi, sh.fin[u]);
+      for (int k = hi - 1; k > e; k--) {
+        if (k < n && k < sh.fin[k % NW]) {
+          const int slot = fr.peak0 + k;
+          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, a.op_peak[slot]);
+          if (lane == 0)
+            for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= a.op_delta[(long long)slot * C_COUNT + q];
+        }
+      }
+      // the op's box becomes the whole frame; it is redone first below
+      if (lane == 0) a.box[fr.peak0 + e] = whole;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    }
+    __syncthreads();
+    from = e;
+  }
+}
+
+template <int PPL, int SHAPE, int NW>
+__global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
+  __shared__ FpwShared<NW> sh;
+  const int f = a.frame_list[blockIdx.x];
+  const Frame fr = a.frames[f];
+  const Model& m = a.m;
+  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int sweeps = 0;
+  for (int it = 0;; it++) {
+    const int n = fpw_list<NW>(a, fr, SEL_RUNNING, sh);
+    if (it > 0 && (n == 0 || it > a.max_more)) break;
+    if (n == 0) continue;  // the unconditional first sweep found nothing
+    sweeps++;
+    // reserved boxes (ROI at the sweep's start + margin) and state backups
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int slot = fr.peak0 + k;
+      const int p = a.op_peak[slot];
+      int4 b;
+      b.x = max(0, a.pk.xi[p] - m.margin);
+      b.y = max(0, a.pk.yi[p] - m.margin);
+      b.z = min(fr.W, a.pk.xi[p] + m.roi + m.margin);
+      b.w = min(fr.H, a.pk.yi[p] + m.roi + m.margin);
+      a.box[slot] = b;
+      for (int q = 0; q < kNFit; q++)
+        a.pk.bk_prm[(long long)q * a.pk.cap + p] = a.pk.prm[(long long)q * a.pk.cap + p];
+      a.pk.bk_lam[p] = a.pk.lam[p];
+      a.pk.bk_err[p] = a.pk.err[p];
+      a.pk.bk_xi[p] = a.pk.xi[p];
+      a.pk.bk_yi[p] = a.pk.yi[p];
+      a.pk.bk_status[p] = a.pk.status[p];
+      a.pk.bk_iter[p] = a.pk.iter[p];
+      a.pk.bk_added[p] = a.pk.added[p];
+    }
+    __syncthreads();
+    fpw_run<PPL, SHAPE, NW>(a, fr, OP_LM, n, sh);
+    // mFitRecenterPeaks (multi_fit.c:1644-1690)
+    const int nr = fpw_list<NW>(a, fr, SEL_RECENTER, sh);
+    if (nr > 0) {
+      for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        const int slot = fr.peak0 + k;
+        const int p = a.op_peak[slot];
+        int nx = (int)floor(a.pk.prm[(long long)P_XC * a.pk.cap + p]);
+        int ny = (int)floor(a.pk.prm[(long long)P_YC * a.pk.cap + p]);
+        int4 b;
+        b.x = max(0, min(fr.W, min(a.pk.xi[p], nx)));
+        b.y = max(0, min(fr.H, min(a.pk.yi[p], ny)));
+        b.z = max(b.x, min(fr.W, max(a.pk.xi[p], nx) + m.roi));
+        b.w = max(b.y, min(fr.H, max(a.pk.yi[p], ny) + m.roi));
+        a.box[slot] = b;
+      }
+      __syncthreads();
+      fpw_run<PPL, SHAPE, NW>(a, fr, OP_RECENTER, nr, sh);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
+    atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
+  if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;
 }
 
 // ---------------------------------------------------------------------------
@@ -1276,18 +1554,18 @@ __global__ __launch_bounds__(256) void k_peak_sums(Model m, Pix px, Peaks pk, co
   PeakState w;
   load_state(pk, p, w);
   Roi<PPL> c;
-  c.init(m, roi_tab);
+  c.init(m, roi_tab, px, fr);
   double t, e;
   lane_shape(m, w, t, e);
   double ps = 0.0, a = 0.0;
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
-    double ex = __shfl(e, c.rx[s], kWave);
-    double ey = __shfl(e, 32 + c.ry[s], kWave);
-    if (c.ok[s]) {
+    double ex = __shfl(e, c.rx(s), kWave);
+    double ey = __shfl(e, 32 + c.ry(s), kWave);
+    if (c.ok(s)) {
       double psf = ey * ex;
       ps += psf;
-      int gx = w.xi + c.rx[s], gy = w.yi + c.ry[s];
+      int gx = w.xi + c.rx(s), gy = w.yi + c.ry(s);
       if (what == 0 && gx >= 0 && gy >= 0 && gx < fr.W && gy < fr.H) {
         long long k = fr.pix0 + (long long)gy * fr.W + gx;
         double b = (px.x[k] - px.scmos[k]) / px.rqe[k] - w.prm[P_HEIGHT] * psf;
@@ -1330,7 +1608,7 @@ __global__ __launch_bounds__(256) void k_peak_jacobian(Model m, Pix px, Peaks pk
     return;
   }
   Roi<PPL> c;
-  c.init(m, roi_tab);
+  c.init(m, roi_tab, px, fr);
   c.load(px, fr, w.xi, w.yi);
   double J[N], H[N * (N + 1) / 2];
   calc_jh<PPL, SHAPE>(m, c, w, px.asx, fr, J, H);
@@ -1483,6 +1761,9 @@ void launch_rollback(const OpArgs& a, hipStream_t st) {
 
 void launch_binning(const Sched& sc, const Frame* frames, const Peaks& pk, int ncells, hipStream_t st) {
   if (sc.nops == 0) return;
+  // cell_fill counts ops per cell here, is zeroed by the scan, and counts
+  // again as the scatter's cursor: clear it first
+  k_fill_i32<<<flat_grid(ncells), 256, 0, st>>>(sc.cell_fill, ncells, 0);
   k_bin_count<<<flat_grid(sc.nops), 256, 0, st>>>(sc, frames, pk);
   k_scan_cells<<<1, 1024, 0, st>>>(sc, ncells);
   k_bin_scatter<<<flat_grid(sc.nops), 256, 0, st>>>(sc, frames, pk);
@@ -1554,6 +1835,45 @@ void launch_fill_i32(int* p, long long n, int v, hipStream_t st) {
 }
 void launch_iota(int* p, int n, int base, hipStream_t st) {
   if (n > 0) k_iota<<<flat_grid(n), 256, 0, st>>>(p, n, base);
+}
+
+// Pack the live peaks of every frame into rows of 9 doubles
+// {x, y, z, height, background, xsigma, ysigma, error, iterations} + status,
+// frame f's peaks starting at row offset[f].
+__global__ void k_pack_peaks(Peaks pk, const Frame* frames, const int* offset, Model m, long long nslots,
+                             double* out, int* status) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  int fi = pk.frame[t];
+  if (fi < 0) return;
+  const Frame fr = frames[fi];
+  long long j = t - fr.peak0;
+  if (j < 0 || j >= fr.nfit) return;
+  long long r = offset[fi] + j;
+  double* o = out + 9 * r;
+  o[0] = pk.prm[(long long)P_XC * pk.cap + t] + m.xoff;
+  o[1] = pk.prm[(long long)P_YC * pk.cap + t] + m.xoff;
+  o[2] = pk.prm[(long long)P_ZC * pk.cap + t] + m.zoff;
+  o[3] = pk.prm[(long long)P_HEIGHT * pk.cap + t];
+  o[4] = pk.prm[(long long)P_BG * pk.cap + t];
+  o[5] = sqrt(1.0 / (2.0 * pk.prm[(long long)P_XW * pk.cap + t]));
+  o[6] = sqrt(1.0 / (2.0 * pk.prm[(long long)P_YW * pk.cap + t]));
+  o[7] = pk.err[t];
+  o[8] = (double)pk.iter[t];
+  status[r] = pk.status[t];
+}
+
+void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, const Model& m, long long nslots,
+                       double* out, int* status, hipStream_t st) {
+  k_pack_peaks<<<flat_grid(nslots), 256, 0, st>>>(pk, frames, offset, m, nslots, out, status);
+}
+
+constexpr int kFpwWaves = 8;
+
+void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st) {
+  if (nframes == 0) return;
+  DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, (k_fit_frames<PPL, SHP, kFpwWaves>
+                                                         <<<nframes, kFpwWaves * 64, 0, st>>>(a))));
 }
 
 }  // namespace dao

@@ -1,0 +1,239 @@
+"""
+Batched fitting: many frames resident on the GPU, fitted together.
+
+The reference fits one frame per MultiFitter (dao_fit_c.py:257-549) and its
+callers loop over frames.  BatchFitter holds a whole set of frames (sizes may
+differ) on the device and runs MultiFitter.doFit (dao_fit_c.py:332-356) for
+all of them in one executor: the per-frame results are those of the
+single-frame fitter, because frames share no state.  Arrays may be numpy
+(host) or torch CUDA tensors (already resident in HBM).
+"""
+import ctypes
+
+import numpy
+from numpy.ctypeslib import ndpointer
+
+from . import loadclib
+from .dao_fit_c import MultiFitterException, formatPeaksGaussianPSF
+
+SHAPES = {"2dfixed": 0, "2d": 1, "3d": 2, "Z": 3}
+ERROR_MODELS = {"MLE": 0, "ALS": 1, "LS": 2, "DWLS": 3, "FWLS": 4}
+COUNTER_NAMES = ("n_dposv", "n_iterations", "n_lost", "n_margin", "n_neg_fi", "n_neg_height",
+                 "n_non_converged", "n_non_decr")
+PEAK_COLUMNS = ("x", "y", "z", "height", "background", "xsigma", "ysigma", "error", "iterations")
+
+
+class daoBatchModel(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int),
+                ("error_model", ctypes.c_int),
+                ("roi_size", ctypes.c_int),
+                ("margin", ctypes.c_int),
+                ("tolerance", ctypes.c_double),
+                ("width_min", ctypes.c_double),
+                ("width_max", ctypes.c_double),
+                ("wx", ctypes.c_double * 5),
+                ("wy", ctypes.c_double * 5),
+                ("z_min", ctypes.c_double),
+                ("z_max", ctypes.c_double)]
+
+
+_lib = None
+
+
+def loadBatchC():
+    global _lib
+    if _lib is None:
+        lib = loadclib.loadCLibrary("dao_fit")
+        vp = ctypes.c_void_p
+        i32p = ndpointer(dtype=numpy.int32)
+        f64p = ndpointer(dtype=numpy.float64)
+        lib.daoBatchCreate.argtypes = [ctypes.POINTER(daoBatchModel), ctypes.c_int, i32p, i32p, i32p]
+        lib.daoBatchCreate.restype = vp
+        lib.daoBatchDestroy.argtypes = [vp]
+        lib.daoBatchSetCalibration.argtypes = [vp, ctypes.c_int, vp, vp]
+        lib.daoBatchNewImages.argtypes = [vp, vp, ctypes.c_int]
+        lib.daoBatchNewBackgrounds.argtypes = [vp, vp, ctypes.c_int]
+        lib.daoBatchAnscombe.argtypes = [vp]
+        lib.daoBatchNewPeaks.argtypes = [vp, vp, i32p, ctypes.c_char_p, ctypes.c_int]
+        lib.daoBatchFit.argtypes = [vp, ctypes.c_int]
+        lib.daoBatchFit.restype = ctypes.c_int
+        lib.daoBatchNFit.argtypes = [vp, ctypes.c_int]
+        lib.daoBatchNFit.restype = ctypes.c_int
+        lib.daoBatchGetPeaks.argtypes = [vp, ctypes.c_int, f64p, i32p]
+        lib.daoBatchGetAllPeaks.argtypes = [vp, f64p, i32p]
+        lib.daoBatchCounters.argtypes = [vp, ctypes.c_int, i32p]
+        lib.daoBatchGetFitImage.argtypes = [vp, ctypes.c_int, f64p]
+        lib.daoBatchRemoveErrorPeaks.argtypes = [vp]
+        lib.daoBatchSetTiming.argtypes = [vp, ctypes.c_int]
+        lib.daoBatchStats.argtypes = [vp, f64p]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, dtype):
+    """(pointer, on_device, keep-alive) for a numpy array or a torch tensor."""
+    if hasattr(a, "data_ptr") and hasattr(a, "is_cuda"):
+        if a.is_cuda:
+            import torch
+            if not a.is_contiguous() or a.dtype != torch.float64:
+                raise MultiFitterException("device arrays must be contiguous float64")
+            # the library works on its own stream: finish torch's writes first
+            torch.cuda.synchronize(a.device)
+            return ctypes.c_void_p(a.data_ptr()), 1, a
+        a = a.numpy()
+    a = numpy.ascontiguousarray(a, dtype=dtype)
+    return a.ctypes.data_as(ctypes.c_void_p), 0, a
+
+
+class BatchFitter(object):
+    """
+    frame_shapes: list of (height, width); peak_caps: max peaks per frame.
+    model: "2dfixed", "2d", "3d" or "Z"; error_model: "MLE", "ALS", "LS",
+    "DWLS", "FWLS".  sigma_range, wx_params, wy_params, min_z, max_z as for
+    the MultiFitter classes.
+    """
+    def __init__(self, frame_shapes, peak_caps, model="Z", error_model="MLE", roi_size=18, sigma_range=None,
+                 wx_params=None, wy_params=None, min_z=None, max_z=None, margin=2, tolerance=1.0e-6):
+        self.clib = loadBatchC()
+        self.shapes = [tuple(s) for s in frame_shapes]
+        self.n_frames = len(self.shapes)
+        m = daoBatchModel()
+        m.shape = SHAPES[model]
+        m.error_model = ERROR_MODELS[error_model]
+        m.roi_size = roi_size
+        m.margin = margin
+        m.tolerance = tolerance
+        if sigma_range is not None:
+            m.width_max = 1.0 / (2.0 * sigma_range[0] * sigma_range[0])
+            m.width_min = 1.0 / (2.0 * sigma_range[1] * sigma_range[1])
+        if model == "Z":
+            for i in range(5):
+                m.wx[i] = wx_params[i]
+                m.wy[i] = wy_params[i]
+            m.z_min, m.z_max = min_z, max_z
+        self.model = model
+        self.error_model = error_model
+        h = numpy.array([s[0] for s in self.shapes], dtype=numpy.int32)
+        w = numpy.array([s[1] for s in self.shapes], dtype=numpy.int32)
+        self.caps = numpy.asarray(peak_caps, dtype=numpy.int32)
+        self.handle = self.clib.daoBatchCreate(ctypes.byref(m), self.n_frames, w, h, self.caps)
+        if not self.handle:
+            raise MultiFitterException("daoBatchCreate failed (no usable GPU, or bad sizes)")
+        self.npix = int(numpy.sum(h.astype(numpy.int64) * w))
+
+    def cleanup(self):
+        if self.handle:
+            self.clib.daoBatchDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+    def setCalibration(self, frame, rqe=None, scmos_cal=None):
+        keep = []
+        args = []
+        for a in (rqe, scmos_cal):
+            if a is None:
+                args.append(None)
+            else:
+                a = numpy.ascontiguousarray(a, dtype=numpy.float64)
+                if a.shape != self.shapes[frame]:
+                    raise MultiFitterException("calibration shape does not match frame %d" % frame)
+                keep.append(a)
+                args.append(a.ctypes.data_as(ctypes.c_void_p))
+        self.clib.daoBatchSetCalibration(self.handle, frame, args[0], args[1])
+
+    def _concat(self, images):
+        if isinstance(images, (list, tuple)):
+            for k, im in enumerate(images):
+                if tuple(im.shape) != self.shapes[k]:
+                    raise MultiFitterException("image %d has the wrong shape" % k)
+            return numpy.concatenate([numpy.ascontiguousarray(im, dtype=numpy.float64).ravel()
+                                      for im in images])
+        n = images.numel() if hasattr(images, "numel") else images.size
+        if n != self.npix:
+            raise MultiFitterException("concatenated images have the wrong size")
+        return images
+
+    def newImages(self, images):
+        p, dev, keep = _ptr(self._concat(images), numpy.float64)
+        self.clib.daoBatchNewImages(self.handle, p, dev)
+        if self.error_model == "ALS":
+            self.clib.daoBatchAnscombe(self.handle)
+
+    def newBackgrounds(self, backgrounds):
+        p, dev, keep = _ptr(self._concat(backgrounds), numpy.float64)
+        self.clib.daoBatchNewBackgrounds(self.handle, p, dev)
+
+    def newPeaks(self, peaks, peaks_type, counts=None):
+        """
+        peaks: a list with one peaks dict per frame (as for
+        MultiFitter.newPeaks), or concatenated rows (numpy / torch) with
+        `counts` giving the rows of each frame.
+        """
+        if isinstance(peaks, (list, tuple)):
+            rows = [formatPeaksGaussianPSF(p, peaks_type) for p in peaks]
+            counts = numpy.array([r.shape[0] for r in rows], dtype=numpy.int32)
+            width = 7 if peaks_type in ("text", "hdf5") else 4
+            rows = numpy.concatenate(rows) if len(rows) else numpy.zeros((0, width))
+        else:
+            rows = peaks
+            if counts is None:
+                raise MultiFitterException("counts are required with concatenated rows")
+        counts = numpy.ascontiguousarray(counts, dtype=numpy.int32)
+        if numpy.any(counts > self.caps):
+            raise MultiFitterException("more peaks than the frame's capacity")
+        p, dev, keep = _ptr(rows, numpy.float64)
+        self.clib.daoBatchNewPeaks(self.handle, p, counts, ctypes.c_char_p(peaks_type.encode()), dev)
+
+    def doFit(self, max_iterations=2000):
+        return self.clib.daoBatchFit(self.handle, max_iterations)
+
+    def removeErrorPeaks(self):
+        self.clib.daoBatchRemoveErrorPeaks(self.handle)
+
+    def getNFit(self, frame):
+        return self.clib.daoBatchNFit(self.handle, frame)
+
+    def getPeaks(self, frame):
+        n = self.getNFit(frame)
+        out = numpy.zeros((n, 9))
+        status = numpy.zeros(n, dtype=numpy.int32)
+        if n:
+            self.clib.daoBatchGetPeaks(self.handle, frame, out, status)
+        d = {c: out[:, k].copy() for k, c in enumerate(PEAK_COLUMNS)}
+        d["iterations"] = d["iterations"].astype(numpy.int32)
+        d["status"] = status
+        return d
+
+    def getAllPeaks(self):
+        """(rows n x 9 in PEAK_COLUMNS order, status, per-frame counts), frames in order."""
+        counts = numpy.array([self.getNFit(f) for f in range(self.n_frames)], dtype=numpy.int64)
+        n = int(counts.sum())
+        out = numpy.zeros((n, 9))
+        status = numpy.zeros(n, dtype=numpy.int32)
+        if n:
+            self.clib.daoBatchGetAllPeaks(self.handle, out, status)
+        return out, status, counts
+
+    def getCounters(self, frame):
+        c = numpy.zeros(8, dtype=numpy.int32)
+        self.clib.daoBatchCounters(self.handle, frame, c)
+        return dict(zip(COUNTER_NAMES, c.tolist()))
+
+    def getFitImage(self, frame):
+        out = numpy.zeros(self.shapes[frame])
+        self.clib.daoBatchGetFitImage(self.handle, frame, out)
+        return out
+
+    def setTiming(self, on=True):
+        self.clib.daoBatchSetTiming(self.handle, 1 if on else 0)
+
+    def stats(self):
+        s = numpy.zeros(6)
+        self.clib.daoBatchStats(self.handle, s)
+        return dict(zip(("lm_ms", "lm_launches", "sweeps", "rounds", "escape_reruns", "max_attempts"),
+                        s.tolist()))

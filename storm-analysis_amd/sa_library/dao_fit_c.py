@@ -120,6 +120,9 @@ def setDaoFitArgtypes(lib):
     lib.mFitGetUnconverged.argtypes = [vp]
     lib.mFitGetUnconverged.restype = ctypes.c_int
     lib.mFitIterateLM.argtypes = [vp]
+    if hasattr(lib, "mFitDoFit"):
+        lib.mFitDoFit.argtypes = [vp, ctypes.c_int]
+        lib.mFitDoFit.restype = ctypes.c_int
     lib.mFitNewBackground.argtypes = [vp, f64]
     lib.mFitNewImage.argtypes = [vp, f64]
     lib.mFitRemoveErrorPeaks.argtypes = [vp]
@@ -230,6 +233,11 @@ class MultiFitter(object):
         One sweep, then more while any peak is RUNNING, at most max_iterations
         more (dao_fit_c.py:332-356).
         """
+        if not self.verbose and hasattr(self.clib, "mFitDoFit"):
+            # the same sweeps in one native call
+            self.clib.mFitDoFit(self.mfit, max_iterations)
+            self.getIterations()
+            return
         i = 0
         self.iterate()
         while self.getUnconverged() and (i < max_iterations):
