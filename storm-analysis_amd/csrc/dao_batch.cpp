@@ -184,25 +184,22 @@ int daoBatchNFit(daoBatch* b, int f) { return b->eng->fr[f].nfit; }
 void daoBatchGetPeaks(daoBatch* b, int f, double* out, int32_t* status) {
   Engine& e = *b->eng;
   int n = e.fr[f].nfit;
-  std::vector<double> v;
-  const int q[7] = {dao::P_XC, dao::P_YC, dao::P_ZC, dao::P_HEIGHT, dao::P_BG, dao::P_XW, dao::P_YW};
-  for (int c = 0; c < 7; c++) {
-    e.get_param(f, q[c], v);
-    for (int i = 0; i < n; i++) {
-      double x = v[i];
-      if (c == 0 || c == 1) x += e.m.xoff;
-      if (c == 2) x += e.m.zoff;
-      if (c >= 5) x = std::sqrt(1.0 / (2.0 * x));
-      out[(size_t)i * 9 + c] = x;
-    }
+  std::vector<dao::PeakRec> r;
+  e.get_records(f, r);
+  for (int i = 0; i < n; i++) {
+    const dao::PeakRec& p = r[i];
+    double* o = out + (size_t)i * 9;
+    o[0] = p.prm[dao::P_XC] + e.m.xoff;
+    o[1] = p.prm[dao::P_YC] + e.m.xoff;
+    o[2] = p.prm[dao::P_ZC] + e.m.zoff;
+    o[3] = p.prm[dao::P_HEIGHT];
+    o[4] = p.prm[dao::P_BG];
+    o[5] = std::sqrt(1.0 / (2.0 * p.prm[dao::P_XW]));
+    o[6] = std::sqrt(1.0 / (2.0 * p.prm[dao::P_YW]));
+    o[7] = p.err;
+    o[8] = (double)p.iter;
+    status[i] = p.status;
   }
-  e.get_double(f, e.pk.err, v);
-  for (int i = 0; i < n; i++) out[(size_t)i * 9 + 7] = v[i];
-  std::vector<int32_t> it, st;
-  e.get_int(f, e.pk.iter, it);
-  for (int i = 0; i < n; i++) out[(size_t)i * 9 + 8] = (double)it[i];
-  e.get_status(f, st);
-  for (int i = 0; i < n; i++) status[i] = st[i];
 }
 
 void daoBatchGetAllPeaks(daoBatch* b, double* out, int32_t* status) { b->eng->pack_all(out, status); }

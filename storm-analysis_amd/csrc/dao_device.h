@@ -1,9 +1,9 @@
 // Device-side data layout shared by the HIP kernels and the host engine.
 //
-// One "batch" holds F frames.  Pixel arrays are structure-of-arrays over all
-// frames (frame f occupies [pix0, pix0 + W*H)), row-major with x the fast
-// axis, pixel index k = (yi + ry) * W + (xi + rx) (multi_fit.c:50).  Peaks are
-// structure-of-arrays; frame f owns slots [peak0, peak0 + cap) and its live
+// One "batch" holds F frames.  Pixel records of all frames share two arrays
+// (frame f occupies [pix0, pix0 + W*H)), row-major with x the fast axis,
+// pixel index k = (yi + ry) * W + (xi + rx) (multi_fit.c:50).  Peak records
+// share one array; frame f owns slots [peak0, peak0 + cap) and its live
 // peaks are the first nfit of them, in the reference's index order.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -60,39 +60,42 @@ struct Frame {
   int ncx, ncy;
 };
 
+// Pixel state, one 32-byte record per pixel.  rw: what the fit changes;
+// ro: the frame's data and calibration.
+struct PixRW {
+  double f;    // fit foreground (rqe applied)
+  double bg;   // sum of covering peaks' bg*rqe + scmos
+  double tfi;  // cached transformed fit value (multi_fit.c:160)
+  int cnt;     // number of peaks covering the pixel
+  int stale;   // fit changed since the error was last evaluated here
+};
+struct PixRO {
+  double x;      // image * rqe + scmos
+  double rqe;
+  double scm;    // variance / gain^2
+  double bgest;  // background estimate * rqe
+};
 struct Pix {
-  double* x;      // image * rqe + scmos
-  double* rqe;
-  double* scmos;  // variance / gain^2
-  double* f;      // fit foreground (rqe applied)
-  double* bg;     // sum of covering peaks' bg*rqe + scmos
-  double* tfi;    // cached transformed fit value per pixel
-  double* bgest;  // background estimate * rqe
-  double* asx;    // Anscombe transform of x (ALS only)
-  int* cnt;       // number of peaks covering the pixel
-  uint8_t* stale; // fit changed since the error was last evaluated here
+  PixRW* rw;
+  PixRO* ro;
+  double* asx;  // Anscombe transform of x (ALS only)
 };
 
+// Peak state, one 96-byte record per peak slot (peakData, multi_fit.h:58-74).
+struct PeakRec {
+  double prm[kNFit];
+  double lam;
+  double err;
+  int xi, yi;
+  int status;
+  int iter;
+  int added;
+  int frame;  // owner frame of the slot
+};
 struct Peaks {
-  double* prm;    // [kNFit][cap_total] param-major
-  double* lam;
-  double* err;
-  int* xi;
-  int* yi;
-  int* status;
-  int* iter;
-  int* added;
-  int* frame;
-  long long cap;  // stride of prm
-  // copy of the state at the start of a sweep attempt (rollback)
-  double* bk_prm;
-  double* bk_lam;
-  double* bk_err;
-  int* bk_xi;
-  int* bk_yi;
-  int* bk_status;
-  int* bk_iter;
-  int* bk_added;
+  PeakRec* rec;
+  PeakRec* bk;  // copy at the start of a sweep (rollback)
+  long long cap;
 };
 
 // The ordered executor: ops i < j of one frame whose boxes overlap run in
@@ -136,6 +139,7 @@ struct FitArgs {
   int* op_peak;         // per-slot scratch: op list of the frame
   int4* box;            // per-slot scratch: op boxes
   int* op_delta;        // per-slot scratch: counter increments of an op
+  int* op_done;         // per-slot scratch: op finished in the current phase
   int* frame_counters;  // [F][C_COUNT]
   int* frame_sweeps;    // [F]: sweeps run by this launch
   const int* frame_list;

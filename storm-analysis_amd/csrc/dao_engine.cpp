@@ -91,24 +91,14 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
   d_fr = dalloc<Frame>(F);
   upload_frames();
 
-  px.x = dalloc<double>(npix);
-  px.rqe = dalloc<double>(npix);
-  px.scmos = dalloc<double>(npix);
-  px.f = dalloc<double>(npix);
-  px.bg = dalloc<double>(npix);
-  px.tfi = dalloc<double>(npix);
-  px.bgest = dalloc<double>(npix);
+  // pixel records start zeroed; every frame gets rqe/scmos from
+  // set_calibration and its fit state from new_image
+  px.rw = dalloc<PixRW>(npix);
+  px.ro = dalloc<PixRO>(npix);
   px.asx = dalloc<double>(npix);
-  px.cnt = dalloc<int>(npix);
-  px.stale = dalloc<uint8_t>(npix);
-  DAO_CHECK(hipMemsetAsync(px.f, 0, sizeof(double) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.bg, 0, sizeof(double) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.tfi, 0, sizeof(double) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.bgest, 0, sizeof(double) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.rw, 0, sizeof(PixRW) * npix, st));
+  DAO_CHECK(hipMemsetAsync(px.ro, 0, sizeof(PixRO) * npix, st));
   DAO_CHECK(hipMemsetAsync(px.asx, 0, sizeof(double) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.x, 0, sizeof(double) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.cnt, 0, sizeof(int) * npix, st));
-  DAO_CHECK(hipMemsetAsync(px.stale, 1, npix, st));
 
   // scheduling grid (cell_start has one extra entry)
   sc.cell = cell;
@@ -130,20 +120,13 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
 
 Engine::~Engine() {
   if (st) (void)hipStreamSynchronize(st);
-  free_peaks(pk);
-  free_peaks(pk2);
+  dfree(pk.rec);
+  dfree(pk.bk);
   free_sched();
   dfree(d_fr);
-  dfree(px.x);
-  dfree(px.rqe);
-  dfree(px.scmos);
-  dfree(px.f);
-  dfree(px.bg);
-  dfree(px.tfi);
-  dfree(px.bgest);
+  dfree(px.rw);
+  dfree(px.ro);
   dfree(px.asx);
-  dfree(px.cnt);
-  dfree(px.stale);
   dfree(sc.cell_start);
   dfree(sc.cell_fill);
   dfree(sc.n_big);
@@ -180,47 +163,6 @@ void Engine::upload_frames() {
   sync();
 }
 
-void Engine::alloc_peaks(Peaks& p, long long c) {
-  p.cap = c;
-  p.prm = dalloc<double>(c * kNFit);
-  p.lam = dalloc<double>(c);
-  p.err = dalloc<double>(c);
-  p.xi = dalloc<int>(c);
-  p.yi = dalloc<int>(c);
-  p.status = dalloc<int>(c);
-  p.iter = dalloc<int>(c);
-  p.added = dalloc<int>(c);
-  p.frame = dalloc<int>(c);
-  p.bk_prm = dalloc<double>(c * kNFit);
-  p.bk_lam = dalloc<double>(c);
-  p.bk_err = dalloc<double>(c);
-  p.bk_xi = dalloc<int>(c);
-  p.bk_yi = dalloc<int>(c);
-  p.bk_status = dalloc<int>(c);
-  p.bk_iter = dalloc<int>(c);
-  p.bk_added = dalloc<int>(c);
-}
-
-void Engine::free_peaks(Peaks& p) {
-  dfree(p.prm);
-  dfree(p.lam);
-  dfree(p.err);
-  dfree(p.xi);
-  dfree(p.yi);
-  dfree(p.status);
-  dfree(p.iter);
-  dfree(p.added);
-  dfree(p.frame);
-  dfree(p.bk_prm);
-  dfree(p.bk_lam);
-  dfree(p.bk_err);
-  dfree(p.bk_xi);
-  dfree(p.bk_yi);
-  dfree(p.bk_status);
-  dfree(p.bk_iter);
-  dfree(p.bk_added);
-}
-
 void Engine::alloc_sched(long long c) {
   sc.op_peak = dalloc<int>(c);
   sc.box = dalloc<int4>(c);
@@ -251,37 +193,31 @@ void Engine::free_sched() {
   dfree(d_flag);
 }
 
-// Grow every per-slot array to c slots, keeping the first `cap` slots.
+// Grow every per-slot array to c slots, keeping the first `cap` records.
+// New slots belong to the frame whose range holds them (the single-frame
+// handle owns everything).
 void Engine::grow_total_cap(long long c) {
   if (c <= cap) return;
   Peaks np{};
-  alloc_peaks(np, c);
-  if (cap > 0) {
-    for (int q = 0; q < kNFit; q++)
-      DAO_CHECK(hipMemcpyAsync(np.prm + q * c, pk.prm + q * cap, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.lam, pk.lam, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.err, pk.err, sizeof(double) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.xi, pk.xi, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.yi, pk.yi, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.status, pk.status, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.iter, pk.iter, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(np.added, pk.added, sizeof(int) * cap, hipMemcpyDeviceToDevice, st));
-  }
+  np.cap = c;
+  np.rec = dalloc<PeakRec>(c);
+  np.bk = dalloc<PeakRec>(c);
+  if (cap > 0) DAO_CHECK(hipMemcpyAsync(np.rec, pk.rec, sizeof(PeakRec) * cap, hipMemcpyDeviceToDevice, st));
   DAO_CHECK(hipStreamSynchronize(st));
-  free_peaks(pk);
-  free_peaks(pk2);
+  dfree(pk.rec);
+  dfree(pk.bk);
   free_sched();
   pk = np;
-  alloc_peaks(pk2, c);
   alloc_sched(c);
+  long long old = cap;
   cap = c;
-  // slot -> frame map: frame f owns [peak0, peak0 + cap_f); the single-frame
-  // handle owns everything
-  std::vector<int> owner(c, F - 1);
-  for (int f = 0; f < F; f++)
-    for (long long s = fr[f].peak0; s < (long long)fr[f].peak0 + fr[f].cap && s < c; s++) owner[s] = f;
-  DAO_CHECK(hipMemcpy(pk.frame, owner.data(), sizeof(int) * c, hipMemcpyHostToDevice));
-  DAO_CHECK(hipMemcpy(pk2.frame, owner.data(), sizeof(int) * c, hipMemcpyHostToDevice));
+  for (int f = 0; f < F; f++) {
+    long long s0 = std::max<long long>(old, fr[f].peak0);
+    long long s1 = std::min<long long>(c, (long long)fr[f].peak0 + fr[f].cap);
+    if (F == 1) s1 = c;
+    launch_set_owner(pk, s0, s1, f, st);
+  }
+  DAO_CHECK(hipStreamSynchronize(st));
 }
 
 void Engine::ensure_frame_cap(int f, int need) {
@@ -332,8 +268,10 @@ void Engine::sync() { DAO_CHECK(hipStreamSynchronize(st)); }
 
 void Engine::set_calibration(int f, const double* rqe_h, const double* scmos_h) {
   long long n = (long long)fr[f].W * fr[f].H;
-  DAO_CHECK(hipMemcpyAsync(px.rqe + fr[f].pix0, rqe_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
-  DAO_CHECK(hipMemcpyAsync(px.scmos + fr[f].pix0, scmos_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  double* d = stage(sizeof(double) * 2 * n);
+  DAO_CHECK(hipMemcpyAsync(d, rqe_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  DAO_CHECK(hipMemcpyAsync(d + n, scmos_h, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  launch_set_calibration(px, fr[f].pix0, n, d, d + n, st);
   sync();
 }
 
@@ -460,6 +398,7 @@ int Engine::fit_frames(int max_more) {
   a.op_peak = sc.op_peak;
   a.box = sc.box;
   a.op_delta = sc.op_delta;
+  a.op_done = sc.op_flags;
   a.frame_counters = d_counters;
   a.frame_sweeps = d_sweeps;
   a.frame_list = d_flist;
@@ -613,19 +552,10 @@ void Engine::compact(int f, const std::vector<int32_t>& keep) {
   if (n > 0) {
     int* d_map = istage(sizeof(int) * n);
     DAO_CHECK(hipMemcpyAsync(d_map, keep.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-    launch_gather_peaks(pk2, pk, d_map, fr[f].peak0, n, st);
-    // copy the compacted range back into pk (other frames' slots untouched)
-    long long s0 = fr[f].peak0;
-    for (int q = 0; q < kNFit; q++)
-      DAO_CHECK(hipMemcpyAsync(pk.prm + q * cap + s0, pk2.prm + q * cap + s0, sizeof(double) * n,
-                               hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.lam + s0, pk2.lam + s0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.err + s0, pk2.err + s0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.xi + s0, pk2.xi + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.yi + s0, pk2.yi + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.status + s0, pk2.status + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.iter + s0, pk2.iter + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
-    DAO_CHECK(hipMemcpyAsync(pk.added + s0, pk2.added + s0, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+    // gather into the backup array, then copy the range back
+    launch_gather_peaks(pk.bk, pk.rec, d_map, fr[f].peak0, n, st);
+    DAO_CHECK(hipMemcpyAsync(pk.rec + fr[f].peak0, pk.bk + fr[f].peak0, sizeof(PeakRec) * n,
+                             hipMemcpyDeviceToDevice, st));
   }
   fr[f].nfit = n;
   upload_frames();
@@ -636,11 +566,11 @@ void Engine::compact(int f, const std::vector<int32_t>& keep) {
 void Engine::set_status(int f, const int32_t* status) {
   int n = fr[f].nfit;
   if (n == 0) return;
-  std::vector<int32_t> added;
-  get_int(f, pk.added, added);
+  std::vector<PeakRec> rec;
+  get_records(f, rec);
   std::vector<int> ops;
   for (int i = 0; i < n; i++)
-    if (status[i] == ST_ERROR && added[i] > 0) ops.push_back(fr[f].peak0 + i);
+    if (status[i] == ST_ERROR && rec[i].added > 0) ops.push_back(fr[f].peak0 + i);
   if (!ops.empty()) {
     int* d = istage(sizeof(int) * ops.size());
     DAO_CHECK(hipMemcpyAsync(d, ops.data(), sizeof(int) * ops.size(), hipMemcpyHostToDevice, st));
@@ -669,25 +599,41 @@ void Engine::read_counters(std::vector<int>& out) {
   out.assign(h, h + n);
 }
 
-void Engine::get_int(int f, const int* arr, std::vector<int32_t>& out) {
+void Engine::get_records(int f, std::vector<PeakRec>& out) {
   int n = fr[f].nfit;
   out.resize(n);
   if (n == 0) return;
-  DAO_CHECK(hipMemcpyAsync(out.data(), arr + fr[f].peak0, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+  DAO_CHECK(hipMemcpyAsync(out.data(), pk.rec + fr[f].peak0, sizeof(PeakRec) * n, hipMemcpyDeviceToHost, st));
   sync();
 }
 
-void Engine::get_status(int f, std::vector<int32_t>& out) { get_int(f, pk.status, out); }
-
-void Engine::get_double(int f, const double* arr, std::vector<double>& out) {
-  int n = fr[f].nfit;
-  out.resize(n);
-  if (n == 0) return;
-  DAO_CHECK(hipMemcpyAsync(out.data(), arr + fr[f].peak0, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  sync();
+void Engine::get_status(int f, std::vector<int32_t>& out) {
+  std::vector<PeakRec> r;
+  get_records(f, r);
+  out.resize(r.size());
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].status;
 }
 
-void Engine::get_param(int f, int q, std::vector<double>& out) { get_double(f, pk.prm + (long long)q * cap, out); }
+void Engine::get_iterations(int f, std::vector<int32_t>& out) {
+  std::vector<PeakRec> r;
+  get_records(f, r);
+  out.resize(r.size());
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].iter;
+}
+
+void Engine::get_error(int f, std::vector<double>& out) {
+  std::vector<PeakRec> r;
+  get_records(f, r);
+  out.resize(r.size());
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].err;
+}
+
+void Engine::get_param(int f, int q, std::vector<double>& out) {
+  std::vector<PeakRec> r;
+  get_records(f, r);
+  out.resize(r.size());
+  for (size_t i = 0; i < r.size(); i++) out[i] = r[i].prm[q];
+}
 
 void Engine::get_sums(int f, int what, std::vector<double>& out) {
   int n = fr[f].nfit;

@@ -30,14 +30,17 @@ void launch_commit(const Sched& sc, const Peaks& pk, int* frame_counters, hipStr
 void launch_collect_escapes(const Sched& sc, const Peaks& pk, int4* override_box, hipStream_t st);
 void launch_new_image(const Pix& px, long long pix0, long long n, const double* img, hipStream_t st);
 void launch_new_background(const Pix& px, long long pix0, long long n, const double* bg, hipStream_t st);
+void launch_set_calibration(const Pix& px, long long pix0, long long n, const double* rqe, const double* scm,
+                            hipStream_t st);
 void launch_anscombe(const Pix& px, long long pix0, long long n, hipStream_t st);
 void launch_fit_image(const Pix& px, long long pix0, long long n, double* out, int residual, hipStream_t st);
 void launch_peak_sums(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames, const int* roi_tab,
                       int slot0, int n, int what, double* out, hipStream_t st);
 void launch_peak_jacobian(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames,
                           const int* roi_tab, int slot0, int n, double* out, hipStream_t st);
-void launch_gather_peaks(const Peaks& dst, const Peaks& src, const int* map, int slot0, int n, hipStream_t st);
+void launch_gather_peaks(PeakRec* dst, const PeakRec* src, const int* map, int slot0, int n, hipStream_t st);
 void launch_set_status(const Peaks& pk, int slot0, int n, const int* status, hipStream_t st);
+void launch_set_owner(const Peaks& pk, long long s0, long long s1, int frame, hipStream_t st);
 void launch_select(const Peaks& pk, const Frame* frames, long long nslots, int mode, const int* frame_bad,
                    int* flag, int* out, int* count, hipStream_t st);
 void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
@@ -70,7 +73,7 @@ class Engine {
   Frame* h_fr = nullptr;  // pinned copy of fr for uploads
   Pix px{};
   long long npix = 0;
-  Peaks pk{}, pk2{};
+  Peaks pk{};
   long long cap = 0;  // total peak slots
   std::vector<int> roi_x, roi_y;
   int* d_roi = nullptr;
@@ -126,9 +129,10 @@ class Engine {
   long long sweeps_total = 0;
   // read-back
   void read_counters(std::vector<int>& out);  // F * C_COUNT, zeroes the device copy
+  void get_records(int f, std::vector<PeakRec>& out);  // frame f's live peak records
   void get_status(int f, std::vector<int32_t>& out);
-  void get_int(int f, const int* arr, std::vector<int32_t>& out);
-  void get_double(int f, const double* arr, std::vector<double>& out);
+  void get_iterations(int f, std::vector<int32_t>& out);
+  void get_error(int f, std::vector<double>& out);
   void get_param(int f, int q, std::vector<double>& out);
   void get_sums(int f, int what, std::vector<double>& out);
   void get_jacobian(int f, std::vector<double>& out);
@@ -143,8 +147,6 @@ class Engine {
   void sync();
 
  private:
-  void alloc_peaks(Peaks& p, long long c);
-  void free_peaks(Peaks& p);
   void alloc_sched(long long c);
   void free_sched();
   void grow_total_cap(long long c);

@@ -247,7 +247,7 @@ void mFitGetPeakPropertyDouble(fitData* fd, double* values, char* what) {
     e.get_sums(0, 0, v);
     put(0.0);
   } else if (!std::strcmp(what, "error")) {
-    e.get_double(0, e.pk.err, v);
+    e.get_error(0, v);
     put(0.0);
   } else if (!std::strcmp(what, "fg_sum") || !std::strcmp(what, "fg_sum_sc")) {
     e.get_sums(0, 1, v);
@@ -285,7 +285,7 @@ void mFitGetPeakPropertyInt(fitData* fd, int32_t* values, char* what) {
   Handle* h = H(fd);
   std::vector<int32_t> v;
   if (!std::strcmp(what, "iterations")) {
-    h->eng->get_int(0, h->eng->pk.iter, v);
+    h->eng->get_iterations(0, v);
   } else if (!std::strcmp(what, "status")) {
     h->eng->get_status(0, v);
   } else {
