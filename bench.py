@@ -35,6 +35,15 @@ HBM_PEAK_GBS = 8000.0
 ROI = 18
 N_ROI_18 = 256
 ALG_BYTES_PER_PIXEL = 36
+# HBM traffic of one batched k_fit_frames launch on the default workload (500
+# frames of 512 x 512, 1000 emitters), from two rocprofv3 --pmc runs of
+# `bench.py --steps 1 --warmup 0 --plain-frames 1 --no-cpu`: FETCH_SIZE and
+# WRITE_SIZE in KiB per dispatch, scaled by what the same counters read for a
+# 1 GiB device copy (`bench.py --copy-calibration 1024`: FETCH_SIZE reads half
+# the bytes moved, WRITE_SIZE all of them).
+MEASURED_TRAFFIC = {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
+                    "fetch_kib": 146414175.9, "write_kib": 105563923.0,
+                    "fetch_scale": 2.0, "write_scale": 1.0}
 
 
 def log(msg):
@@ -50,10 +59,25 @@ def parse():
     p.add_argument("--emitters", type=int, default=1000)
     p.add_argument("--size", type=int, default=512)
     p.add_argument("--plain-frames", type=int, default=20)
-    p.add_argument("--cpu-frames", type=int, default=32)
+    p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device-to-device copy of MIB MiB three times and exit "
+                        "(the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied %d MiB three times (%d bytes read and %d written per copy)" % (mib, 4 * n, 4 * n))
 
 
 def zkw():
@@ -71,8 +95,10 @@ def peaks_dict(r):
             "xsigma": r[:, 5], "ysigma": r[:, 6]}
 
 
-def fit_plain(cls_factory, image, background, rows):
+def fit_plain(cls_factory, image, background, rows, lib=None):
     m = cls_factory()
+    if lib is not None and m.clib is not lib:
+        raise RuntimeError("fitter is not bound to the expected library")
     m.initializeC(image)
     m.newImage(image)
     m.newBackground(background)
@@ -93,15 +119,18 @@ def cpu_baseline(images, backgrounds, rows, counts, nframes):
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
 
+    fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
+
     def one(f):
-        with mock.patch.object(D, "loadDaoFitC", lambda: lib):
-            fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
-        out = fit_plain(fac, images[f], backgrounds[f], frame_rows(rows, counts, f))
+        out = fit_plain(fac, images[f], backgrounds[f], frame_rows(rows, counts, f), lib=lib)
         return int((out["status"] == D.CONVERGED).sum()), out
 
-    t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(cores) as ex:
-        res = list(ex.map(one, range(nframes)))
+    # Every fitter built while the patch is active binds the oracle library;
+    # the patch stays in place until all threads have finished.
+    with mock.patch.object(D, "loadDaoFitC", lambda: lib):
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(cores) as ex:
+            res = list(ex.map(one, range(nframes)))
     dt = time.perf_counter() - t0
     conv = sum(r[0] for r in res)
     return {"value": conv / dt, "unit": "converged fits/s", "cores": cores, "kind": "port",
@@ -115,6 +144,9 @@ def main():
         print("bench.py measures one GPU: the fitter has no multi-GPU path (frames would be "
               "sharded across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     import torch
     __graft_entry__.load_package()
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
@@ -166,6 +198,10 @@ def main():
     alg_bytes = float(cycles) * N_ROI_18 * ALG_BYTES_PER_PIXEL
     achieved = alg_bytes / (lm_ms * 1e-3) / 1e9 if lm_ms > 0 else 0.0
 
+    traffic = None
+    mt = MEASURED_TRAFFIC
+    if (F, S, N, args.margin) == (mt["frames"], mt["size"], mt["emitters"], mt["margin"]):
+        traffic = 1024.0 * (mt["fetch_kib"] * mt["fetch_scale"] + mt["write_kib"] * mt["write_scale"])
     log("batched: %.1f ms/step, %d converged per step" % (dt * 1e3 / args.steps, conv // max(1, args.steps)))
     # plain path: one frame at a time through the single-frame C-ABI
     host_images = images[:args.plain_frames].cpu().numpy()
@@ -210,7 +246,7 @@ def main():
         cpu, cpu_out = cpu_baseline(host_images_cpu, host_bgs_cpu, host_rows, counts_np, nfr)
         # GPU batched vs the oracle on the same frames
         max_d = 0.0
-        for f in range(min(nfr, args.plain_frames)):
+        for f in range(nfr):
             b = out[c[f]:c[f + 1]]
             st = status[c[f]:c[f + 1]]
             flags_equal &= bool(numpy.array_equal(st, cpu_out[f]["status"]))
@@ -238,8 +274,10 @@ def main():
                                (F, S, S, N, ROI),
                    "frames": F, "frame_size": S, "emitters_per_frame": N, "model": "Z", "roi_size": ROI},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_lm (one LM turn per peak)", "lm_ms": lm_ms, "lm_launches": lm_launches,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_unit": "bytes per launch (rocprofv3 --pmc, copy-scaled)",
+                     "alg_bytes_per_launch": alg_bytes / max(1, lm_launches),
+                     "kernel": "k_fit_frames (persistent frame-per-workgroup doFit)", "lm_ms": lm_ms, "lm_launches": lm_launches,
                      "avg_launch_us": (lm_ms * 1e3 / lm_launches) if lm_launches else None,
                      "lm_cycles": cycles, "bytes_per_cycle": N_ROI_18 * ALG_BYTES_PER_PIXEL},
         "cpu_baseline": cpu,
