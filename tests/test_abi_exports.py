@@ -1,0 +1,49 @@
+"""
+The C-ABI library loads without a GPU and exports every function that
+include/*.h declares (no compute calls: those need a GPU).
+"""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "storm-analysis_amd", "c_libraries", "libdao_fit.so")
+DECL = re.compile(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*\**([A-Za-z_][A-Za-z0-9_]*)\s*\(", re.M)
+
+
+def declared():
+    names = []
+    inc = os.path.join(ROOT, "include")
+    for fn in sorted(os.listdir(inc)):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(inc, fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)  # drop comments
+        text = re.sub(r"typedef\s+struct[^;]*?\{.*?\}\s*\w+\s*;", "", text, flags=re.S)
+        for m in DECL.finditer(text):
+            if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
+                names.append((fn, m.group(1)))
+    return names
+
+
+def test_headers_declare_the_reference_surface():
+    names = {n for _, n in declared()}
+    # the reference's ctypes binding (storm_analysis/sa_library/dao_fit_c.py:147-238)
+    for n in ("daoInitialize", "daoInitialize2DFixed", "daoInitialize2DFixedALS", "daoInitialize2DFixedLS",
+              "daoInitialize2DFixedDWLS", "daoInitialize2DFixedFWLS", "daoInitialize2D", "daoInitialize2DALS",
+              "daoInitialize2DLS", "daoInitialize2DDWLS", "daoInitialize2DFWLS", "daoInitialize3D",
+              "daoInitializeZ", "daoNewPeaks", "daoCleanup", "mFitNewImage", "mFitNewBackground",
+              "mFitAnscombeTransformImage", "mFitIterateLM", "mFitGetUnconverged", "mFitGetNError",
+              "mFitGetFitImage", "mFitGetResidual", "mFitGetPeakPropertyDouble", "mFitGetPeakPropertyInt",
+              "mFitRemoveErrorPeaks", "mFitRemoveRunningPeaks", "mFitSetPeakStatus"):
+        assert n in names, n
+
+
+@pytest.mark.parametrize("header,name", declared())
+def test_library_exports(header, name):
+    if not os.path.exists(LIB):
+        pytest.fail("%s is not built (run __graft_entry__.build())" % LIB)
+    lib = ctypes.CDLL(LIB)
+    assert hasattr(lib, name), "%s declared in include/%s is not exported" % (name, header)
