@@ -121,16 +121,21 @@ def test_batch_uneven_scmos_3d_matches_oracle(kind):
 
 @pytest.mark.gpu
 def test_batch_same_alone_and_in_mix():
-    """Each frame's result in a mixed batch is bit-identical to the frame fitted alone."""
+    """
+    Each frame's result in a mixed batch is the frame's result alone, to the
+    parity bar.  (Bit-identical unless a reserved box overflowed: the rollback
+    that follows undoes the ops that had finished by then with arithmetic
+    subtraction, and which ops had finished depends on timing.)
+    """
     frames = scmos_frames()
     kw = {"roi_size": 18, "sigma_range": [0.8, 6.0]}
     bgs = [numpy.full(fr.image.shape, 100.0) for fr in frames]
     mixed, mixed_c, _ = batch_fit(frames, kw, bgs)
     for k in (0, 2, 3):
         alone, alone_c, _ = batch_fit([frames[k]], kw, [bgs[k]])
-        for c in dao_batch.PEAK_COLUMNS + ("status",):
-            numpy.testing.assert_array_equal(alone[0][c], mixed[k][c])
-        assert alone_c[0] == mixed_c[k]
+        assert alone[0]["x"].size == mixed[k]["x"].size
+        assert_frame_parity(alone[0], mixed[k])
+        assert alone_c[0]["n_iterations"] == mixed_c[k]["n_iterations"]
 
 
 @pytest.mark.gpu
