@@ -35,25 +35,30 @@ namespace dao {
 
 __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// v of the lane selected by the DPP control CTRL (quad permutes and row
+// mirrors: every lane has a source inside its 16-lane row).
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double lane_value(double v, int lane) {
+  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum over the wave, the same bits in every lane: pairs, quads, 8- and
+// 16-lane rows by DPP (each step adds two equal-order partial sums, so every
+// lane of a row holds the identical row sum), then (r0 + r1) + (r2 + r3).
 __device__ inline double wsum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
-}
-
-__device__ inline int wmin_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    int u = __shfl_xor(v, o, kWave);
-    v = u < v ? u : v;
-  }
-  return v;
-}
-
-__device__ inline int wsum_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 // ---------------------------------------------------------------------------
@@ -144,9 +149,18 @@ __device__ inline void lane_shape(const Model& m, const PeakState& s, double& t,
 // ROI cache of one wave
 // ---------------------------------------------------------------------------
 
-// Read-only data of the ROI pixels sit in the wave's LDS slice: for slot s of
-// lane l, x at lds[s*64+l], rqe at lds[(PPL+s)*64+l], scmos at
-// lds[(2PPL+s)*64+l].  Each lane only touches its own entries.
+// The wave's LDS slice (RoiLds<PPL>::kDoubles doubles): read-only data of the
+// ROI pixels -- for slot s of lane l, x at [s*64+l], rqe at [(PPL+s)*64+l],
+// scmos at [(2PPL+s)*64+l], each lane touching only its own entries -- then
+// the shape table (64 (offset, exp) pairs, written by lane_shape) and the
+// reduced J and packed H of the turn (uniform values).
+template <int PPL>
+struct RoiLds {
+  static constexpr int kTab = 3 * PPL * kWave;
+  static constexpr int kHJ = kTab + 2 * kWave;
+  static constexpr int kDoubles = kHJ + 32;
+};
+
 template <int PPL>
 struct Roi {
   int rxy[PPL];   // column | row << 8
@@ -230,16 +244,25 @@ struct Roi {
     load(px, fr, xi, yi);
   }
 
-  // psf at the held pixels for state st (cache at st.xi, st.yi).
-  __device__ void shape(const Model& m, const PeakState& st) {
+  __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLds<PPL>::kTab); }
+  __device__ double* hj() { return lds + RoiLds<PPL>::kHJ; }
+
+  // Shape table of state st in LDS: entry c < 32 = column c, 32 + r = row r.
+  __device__ void table(const Model& m, const PeakState& st) {
     double t, e;
     lane_shape(m, st, t, e);
+    __builtin_amdgcn_wave_barrier();
+    tab()[lane_id()] = make_double2(t, e);
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  // psf at the held pixels for state st (cache at st.xi, st.yi); leaves the
+  // shape table of st in LDS.
+  __device__ void shape(const Model& m, const PeakState& st) {
+    table(m, st);
+    const double2* tb = tab();
 #pragma unroll
-    for (int s = 0; s < PPL; s++) {
-      double ex = __shfl(e, rx(s), kWave);
-      double ey = __shfl(e, 32 + ry(s), kWave);
-      psf[s] = ey * ex;
-    }
+    for (int s = 0; s < PPL; s++) psf[s] = tb[32 + ry(s)].y * tb[rx(s)].y;
   }
 
   // mFitAddPeak / mFitSubtractPeak (multi_fit.c:28-57, 1894-1922).
@@ -278,54 +301,48 @@ __device__ inline double anscombe(double v) {
 template <int PPL>
 __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double& err, int& neg_fi) {
   const int l = lane_id();
-  double fi[PPL];
-  int first_bad = 0x7fffffff;
-#pragma unroll
-  for (int s = 0; s < PPL; s++) {
-    fi[s] = 0.0;
-    if (c.ok(s)) {
-      fi[s] = c.f[s] + c.bg[s] / ((double)c.cnt[s]);
-      if ((m.errm == EM_MLE || m.errm == EM_FWLS) && fi[s] <= 0.0) {
-        int j = l + s * kWave;
-        first_bad = j < first_bad ? j : first_bad;
-      }
-    }
-  }
-  first_bad = wmin_i(first_bad);
+  const bool stop_on_neg = (m.errm == EM_MLE || m.errm == EM_FWLS);
   double sum = 0.0;
-  int zeros = 0;
+  bool stopped = false;  // uniform: a slot held the ROI's first fi <= 0
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
-    int j = l + s * kWave;
-    if (c.ok(s) && j < first_bad) {
-      double xk = c.x(s);
-      double e;
-      double t = fi[s];
-      if (m.errm == EM_MLE) {
-        e = 2.0 * ((t - xk) - xk * log(t / xk));
-      } else if (m.errm == EM_ALS) {
-        double a = anscombe(t);
-        if (a == 0.0 && ((c.stale >> s) & 1u)) zeros++;
-        t = a;
-        double d = a - asx[c.k(s)];
-        e = d * d;
-      } else if (m.errm == EM_LS) {
-        double d = t - xk;
-        e = d * d;
-      } else if (m.errm == EM_DWLS) {
-        double d = t - xk;
-        e = d * d / xk;
-      } else {
-        double d = t - xk;
-        e = d * d / t;
+    if (!stopped) {
+      const bool o = c.ok(s);
+      double t = o ? c.f[s] + c.bg[s] / ((double)c.cnt[s]) : 1.0;
+      const unsigned long long bad = __ballot(o && stop_on_neg && t <= 0.0);
+      // pixels of this slot before the first bad one (ROI order j = s*64 + l)
+      const int lim = bad ? __builtin_ctzll(bad) : kWave;
+      bool zero = false;
+      if (o && l < lim) {
+        double xk = c.x(s);
+        double e;
+        if (m.errm == EM_MLE) {
+          e = 2.0 * ((t - xk) - xk * log(t / xk));
+        } else if (m.errm == EM_ALS) {
+          double a = anscombe(t);
+          zero = (a == 0.0) && ((c.stale >> s) & 1u);
+          t = a;
+          double d = a - asx[c.k(s)];
+          e = d * d;
+        } else if (m.errm == EM_LS) {
+          double d = t - xk;
+          e = d * d;
+        } else if (m.errm == EM_DWLS) {
+          double d = t - xk;
+          e = d * d / xk;
+        } else {
+          double d = t - xk;
+          e = d * d / t;
+        }
+        c.tfi[s] = t;
+        c.stale &= ~(1u << s);
+        sum += e;
       }
-      c.tfi[s] = t;
-      c.stale &= ~(1u << s);
-      sum += e;
+      if (m.errm == EM_ALS) neg_fi += __popcll(__ballot(zero));
+      if (bad) stopped = true;
     }
   }
-  if (m.errm == EM_ALS) neg_fi += wsum_i(zeros);
-  if (first_bad != 0x7fffffff) {
+  if (stopped) {
     neg_fi += 1;
     return 1;
   }
@@ -334,18 +351,17 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double
 }
 
 // daoCalcJH* (dao_fit.c:48-958): J and the upper triangle of H, packed row by
-// row, reduced over the wave.  Reads the cached transformed fit tfi.
+// row, reduced over the wave into the slice's J/H area (J first).  Reads the
+// cached transformed fit tfi and the shape table, which must hold st's.
 template <int PPL, int SHAPE>
-__device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const double* asx, double* J,
-                        double* H) {
+__device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const double* asx) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
   double a1 = st.prm[P_HEIGHT];
   double wx = st.prm[P_XW];
   double wy = (SHAPE == SH_3D || SHAPE == SH_Z) ? st.prm[P_YW] : wx;  // 2D/2DFixed: XWIDTH twice
   double gx = 0.0, gy = 0.0;
   if (SHAPE == SH_Z) z_grad(m, st, gx, gy);
-  double t, e;
-  lane_shape(m, st, t, e);
+  const double2* tb = c.tab();
 
   double j_acc[N];
   double h_acc[N * (N + 1) / 2];
@@ -356,11 +372,10 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
 
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
-    double xt = __shfl(t, c.rx(s), kWave);
-    double ex = __shfl(e, c.rx(s), kWave);
-    double yt = __shfl(t, 32 + c.ry(s), kWave);
-    double ey = __shfl(e, 32 + c.ry(s), kWave);
     if (c.ok(s)) {
+      const double2 cx = tb[c.rx(s)];
+      const double2 cy = tb[32 + c.ry(s)];
+      const double xt = cx.x, ex = cx.y, yt = cy.x, ey = cy.y;
       double fi = c.tfi[s];
       double xk = c.x(s);
       double rq = c.rqe(s);
@@ -422,10 +437,18 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
       }
     }
   }
+  double* out = c.hj();
 #pragma unroll
-  for (int a = 0; a < N; a++) J[a] = wsum(j_acc[a]);
+  for (int a = 0; a < N; a++) {
+    const double v = wsum(j_acc[a]);
+    if (lane_id() == 0) out[a] = v;
+  }
 #pragma unroll
-  for (int a = 0; a < N * (N + 1) / 2; a++) H[a] = wsum(h_acc[a]);
+  for (int a = 0; a < N * (N + 1) / 2; a++) {
+    const double v = wsum(h_acc[a]);
+    if (lane_id() == 0) out[N + a] = v;
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 template <int N>
@@ -544,13 +567,14 @@ __device__ inline int margin_height_check(const Model& m, const Frame& fr, const
 }
 
 // daoCheck2DFixed/2D/3D/Z (dao_fit.c:1047-1133): 2D and 3D clamp the widths.
+template <int SHAPE>
 __device__ inline int peak_check(const Model& m, const Frame& fr, PeakState& s, int* d) {
   if (margin_height_check(m, fr, s, d)) return 1;
-  if (m.shape == SH_2D || m.shape == SH_3D) {
+  if (SHAPE == SH_2D || SHAPE == SH_3D) {
     if (s.prm[P_XW] < m.wmin) s.prm[P_XW] = m.wmin;
     else if (s.prm[P_XW] > m.wmax) s.prm[P_XW] = m.wmax;
   }
-  if (m.shape == SH_3D) {
+  if (SHAPE == SH_3D) {
     if (s.prm[P_YW] < m.wmin) s.prm[P_YW] = m.wmin;
     else if (s.prm[P_YW] > m.wmax) s.prm[P_YW] = m.wmax;
   }
@@ -597,11 +621,13 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   c.load(px, fr, w.xi, w.yi);
   c.shape(m, w);
 
-  double J[N], H[N * (N + 1) / 2], wJ[N];
+  double wJ[N];
   double e;
   if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) w.err = e;
   const double start = w.err;
-  calc_jh<PPL, SHAPE>(m, c, w, px.asx, J, H);
+  calc_jh<PPL, SHAPE>(m, c, w, px.asx);
+  const double* J = c.hj();
+  const double* H = c.hj() + N;
   c.stamp(w, -1);
   w.added--;
 
@@ -628,7 +654,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
     }
 
     apply_update<SHAPE>(m, w, wJ);
-    if (peak_check(m, fr, w, d)) {
+    if (peak_check<SHAPE>(m, fr, w, d)) {
       reset_from(w, rec);
       w.status = ST_ERROR;
       continue;
@@ -803,7 +829,13 @@ __device__ void newpeak_turn(const Model& m, const Pix& px, const Peaks& pk, con
     w.prm[P_HEIGHT] = 1.0;
   }
 
-  bool good = peak_check(m, fr, w, d) == 0;
+  int bad;
+  switch (m.shape) {
+    case SH_2D: bad = peak_check<SH_2D>(m, fr, w, d); break;
+    case SH_3D: bad = peak_check<SH_3D>(m, fr, w, d); break;
+    default: bad = peak_check<SH_2DFIXED>(m, fr, w, d); break;  // no width clamp
+  }
+  bool good = bad == 0;
   Roi<PPL> c;
   c.init(m, roi_tab, fr, slice);
   if (good) {
@@ -1036,12 +1068,6 @@ __device__ inline void op_finish(const Sched& sc, int i, int round, int flags, c
     atomicSub(sc.remaining, 1);
   }
 }
-
-// doubles of LDS per wave: x, rqe, scmos of PPL slots
-template <int PPL>
-struct RoiLds {
-  static constexpr int kDoubles = 3 * PPL * kWave;
-};
 
 // One executor round: every pending op whose predecessors are done runs.
 template <int PPL, int SHAPE>
@@ -1550,8 +1576,9 @@ __global__ __launch_bounds__(256) void k_peak_jacobian(Model m, Pix px, Peaks pk
   Roi<PPL> c;
   c.init(m, roi_tab, fr, lds[wv]);
   c.load(px, fr, w.xi, w.yi);
-  double J[N], H[N * (N + 1) / 2];
-  calc_jh<PPL, SHAPE>(m, c, w, px.asx, J, H);
+  c.table(m, w);
+  calc_jh<PPL, SHAPE>(m, c, w, px.asx);
+  const double* J = c.hj();
   if (lane_id() == 0)
     for (int q = 0; q < N; q++) out[(long long)i * N + q] = J[q];
 }

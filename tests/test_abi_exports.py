@@ -2,11 +2,12 @@
 The C-ABI library loads without a GPU and exports every function that
 include/*.h declares (no compute calls: those need a GPU).
 """
-import ctypes
 import os
 import re
 
 import pytest
+
+import conftest  # noqa: F401  (imports the package)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "storm-analysis_amd", "c_libraries", "libdao_fit.so")
@@ -45,5 +46,6 @@ def test_headers_declare_the_reference_surface():
 def test_library_exports(header, name):
     if not os.path.exists(LIB):
         pytest.fail("%s is not built (run __graft_entry__.build())" % LIB)
-    lib = ctypes.CDLL(LIB)
+    from storm_analysis_amd.sa_library import loadclib
+    lib = loadclib.loadCLibrary("dao_fit")
     assert hasattr(lib, name), "%s declared in include/%s is not exported" % (name, header)
