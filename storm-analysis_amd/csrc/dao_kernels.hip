@@ -406,13 +406,15 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
         g[3] = -a1 * xt * xt * gx * et - a1 * yt * yt * gy * et;
         g[4] = 1.0;
       }
-      double sc = (m.errm == EM_ALS) ? rq * 2.0 / fi : rq;
+      const double rf = 1.0 / fi;
+      double sc = (m.errm == EM_ALS) ? rq * 2.0 * rf : rq;
 #pragma unroll
       for (int a = 0; a < N; a++) g[a] *= sc;
       double wj, wh;
       if (m.errm == EM_MLE) {
-        wj = 2.0 * (1.0 - xk / fi);
-        wh = 2.0 * xk / (fi * fi);
+        const double xr = xk * rf;
+        wj = 2.0 * (1.0 - xr);
+        wh = 2.0 * xr * rf;
       } else if (m.errm == EM_ALS) {
         wj = fi - asx[c.k(s)];
         wh = 1.0;
@@ -420,13 +422,13 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
         wj = fi - xk;
         wh = 1.0;
       } else if (m.errm == EM_DWLS) {
-        wj = (fi - xk) / xk;
-        wh = 1.0 / xk;
+        const double rx = 1.0 / xk;
+        wj = (fi - xk) * rx;
+        wh = rx;
       } else {
-        double t1 = (fi - xk) / fi;
-        double t2 = 1.0 / fi;
+        double t1 = (fi - xk) * rf;
         wj = 2.0 * t1 - t1 * t1;
-        wh = 2.0 * t2 * (xk * t2 - t1 + t1 * t1);
+        wh = 2.0 * rf * (xk * rf - t1 + t1 * t1);
       }
       int q = 0;
 #pragma unroll
@@ -463,20 +465,21 @@ __device__ constexpr int upk(int r, int c) {  // packed index of (r, c), r <= c
 template <int N>
 __device__ inline int chol_solve(const double* H, double lam, double* b) {
   double L[N][N];
+  double inv[N];  // 1 / L[j][j]
 #pragma unroll
   for (int j = 0; j < N; j++) {
     double s = (1.0 + lam) * H[upk<N>(j, j)];
 #pragma unroll
     for (int k = 0; k < j; k++) s -= L[j][k] * L[j][k];
     if (!(s > 0.0)) return j + 1;
-    double d = sqrt(s);
-    L[j][j] = d;
+    const double d = sqrt(s);
+    inv[j] = 1.0 / d;
 #pragma unroll
     for (int i = j + 1; i < N; i++) {
       double t = H[upk<N>(j, i)];
 #pragma unroll
       for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
-      L[i][j] = t / d;
+      L[i][j] = t * inv[j];
     }
   }
 #pragma unroll
@@ -484,14 +487,14 @@ __device__ inline int chol_solve(const double* H, double lam, double* b) {
     double t = b[i];
 #pragma unroll
     for (int k = 0; k < i; k++) t -= L[i][k] * b[k];
-    b[i] = t / L[i][i];
+    b[i] = t * inv[i];
   }
 #pragma unroll
   for (int i = N - 1; i >= 0; i--) {
     double t = b[i];
 #pragma unroll
     for (int k = i + 1; k < N; k++) t -= L[k][i] * b[k];
-    b[i] = t / L[i][i];
+    b[i] = t * inv[i];
   }
 #pragma unroll
   for (int i = 0; i < N; i++)
