@@ -4,17 +4,22 @@ Benchmark: converged emitter fits/s of the MI355X 3D-DAOSTORM fitter, batched
 vs plain, with frame latency, on the 3D-astigmatism workload: 500 frames of
 512 x 512, 1000 emitters per frame, "Z" model (wx = [2.0 px, +0.15 um, 0.4 um,
 0, 0], wy = [2.0, -0.15, 0.4, 0, 0], z in [-0.5, 0.5] um), ROI 18 px, MLE,
-FP64 state, tolerance 1e-6.
+tolerance 1e-6.
 
 One step = newImage + newBackground + newPeaks ("hdf5" start peaks) + doFit
 (until no peak is RUNNING, at most 2000 sweeps) + all peak properties back on
 the host, for every frame of the batch; images and start peaks are resident
-in HBM when the timed region starts.  `value` is batched converged fits/s.
-Also reported: the plain path (MultiFitterZ, one frame at a time through the
-single-frame C-ABI, host arrays in and out) with its p50/p99 frame latency,
-the LM kernel's HBM roofline (algorithmic bytes = cycles x N_roi x 36 B over
-the LM kernel's event-timed duration), and the oracle timed on the host's
-cores on a sample of the same frames.
+in HBM when the timed region starts.
+
+`value` is the batched converged fits/s in mixed precision (FP32 deviance
+terms, Jacobian and Hessian; FP64 peak shape, fit-image accumulators,
+deviance sums, Cholesky solve and accept/reject), whose accuracy against FP64 is
+reported beside it (all frames against the FP64 GPU path, the CPU sample
+against the oracle).  Also reported: the batched FP64 path, the plain path
+(MultiFitterZ, one frame at a time through the single-frame C-ABI, FP64, host
+arrays in and out) with its p50/p99 frame latency, the fit kernel's HBM
+roofline (algorithmic bytes = LM cycles x N_roi x 36 B over the kernel's
+event-timed duration), and the oracle timed on the host's cores.
 """
 import argparse
 import concurrent.futures as cf
@@ -36,14 +41,18 @@ ROI = 18
 N_ROI_18 = 256
 ALG_BYTES_PER_PIXEL = 36
 # HBM traffic of one batched k_fit_frames launch on the default workload (500
-# frames of 512 x 512, 1000 emitters), from two rocprofv3 --pmc runs of
-# `bench.py --steps 1 --warmup 0 --plain-frames 1 --no-cpu`: FETCH_SIZE and
-# WRITE_SIZE in KiB per dispatch, scaled by what the same counters read for a
-# 1 GiB device copy (`bench.py --copy-calibration 1024`: FETCH_SIZE reads half
-# the bytes moved, WRITE_SIZE all of them).
-MEASURED_TRAFFIC = {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
-                    "fetch_kib": 146414175.9, "write_kib": 105563923.0,
-                    "fetch_scale": 2.0, "write_scale": 1.0}
+# frames of 512 x 512, 1000 emitters), per precision, from two rocprofv3 --pmc
+# runs of `bench.py --steps 1 --warmup 0 --plain-frames 1 --no-cpu
+# --precision P`: FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by
+# what the same counters read for a 1 GiB device copy (`bench.py
+# --copy-calibration 1024`: FETCH_SIZE reads half the bytes moved, WRITE_SIZE
+# all of them).
+MEASURED_TRAFFIC = {
+    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
+             "fetch_kib": 146414175.9, "write_kib": 105563923.0, "fetch_scale": 2.0, "write_scale": 1.0},
+}
+# accuracy bar of mixed precision against FP64 for x and y
+TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
 
 
 def log(msg):
@@ -62,6 +71,8 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
+    p.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
+                   help="batched precisions to time (value = mixed when timed)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device-to-device copy of MIB MiB three times and exit "
                         "(the known-size kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -138,30 +149,13 @@ def cpu_baseline(images, backgrounds, rows, counts, nframes):
                       (nframes, conv, dt, cores)}, [r[1] for r in res]
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the fitter has no multi-GPU path (frames would be "
-              "sharded across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    if args.copy_calibration > 0:
-        copy_calibration(args.copy_calibration)
-        return
+def run_batched(precision, args, images, backgrounds, rows, counts_np):
+    """Time args.steps batched steps in one precision; returns the measurements and results."""
     import torch
-    __graft_entry__.load_package()
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
-    from storm_analysis_amd.diagnostics import synth_gpu
-
-    dev = "cuda:0"
     F, S, N = args.frames, args.size, args.emitters
-    t_gen = time.perf_counter()
-    log("generating %d frames of %dx%d with %d emitters on %s" % (F, S, S, N, dev))
-    images, backgrounds, rows, counts, truth = synth_gpu.frames_z(F, size=S, n=N, device=dev)
-    torch.cuda.synchronize()
-    t_gen = time.perf_counter() - t_gen
-
-    bf = dao_batch.BatchFitter([(S, S)] * F, [N] * F, model="Z", roi_size=ROI, margin=args.margin, **zkw())
-    counts_np = counts.numpy()
+    bf = dao_batch.BatchFitter([(S, S)] * F, [N] * F, model="Z", roi_size=ROI, margin=args.margin,
+                               precision=precision, **zkw())
 
     def step():
         bf.newImages(images)
@@ -170,10 +164,9 @@ def main():
         bf.doFit()
         return bf.getAllPeaks()
 
-    log("batch of %d frames created; warmup %d step(s)" % (F, args.warmup))
+    log("%s: batch of %d frames created; warmup %d step(s)" % (precision, F, args.warmup))
     for _ in range(args.warmup):
         step()
-    log("timing %d step(s)" % args.steps)
     for f in range(F):
         bf.getCounters(f)
     bf.setTiming(True)
@@ -187,31 +180,95 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     s1 = bf.stats()
-    cycles = 0
-    escapes = 0
-    for f in range(F):
-        c = bf.getCounters(f)
-        cycles += c["n_iterations"]
-    batched_fps = conv / dt
+    cycles = sum(bf.getCounters(f)["n_iterations"] for f in range(F))
     lm_ms = s1["lm_ms"] - s0["lm_ms"]
-    lm_launches = s1["lm_launches"] - s0["lm_launches"]
+    launches = s1["lm_launches"] - s0["lm_launches"]
     alg_bytes = float(cycles) * N_ROI_18 * ALG_BYTES_PER_PIXEL
     achieved = alg_bytes / (lm_ms * 1e-3) / 1e9 if lm_ms > 0 else 0.0
-
     traffic = None
-    mt = MEASURED_TRAFFIC
-    if (F, S, N, args.margin) == (mt["frames"], mt["size"], mt["emitters"], mt["margin"]):
+    mt = MEASURED_TRAFFIC.get(precision)
+    if mt and (F, S, N, args.margin) == (mt["frames"], mt["size"], mt["emitters"], mt["margin"]):
         traffic = 1024.0 * (mt["fetch_kib"] * mt["fetch_scale"] + mt["write_kib"] * mt["write_scale"])
-    log("batched: %.1f ms/step, %d converged per step" % (dt * 1e3 / args.steps, conv // max(1, args.steps)))
-    # plain path: one frame at a time through the single-frame C-ABI
+    st = bf.stats()
+    bf.cleanup()
+    log("%s: %.1f ms/step, %d converged per step, fit kernel %.1f ms per launch" %
+        (precision, dt * 1e3 / args.steps, conv // max(1, args.steps), lm_ms / max(1, launches)))
+    return {
+        "fits_per_s": conv / dt,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "converged_per_step": conv // max(1, args.steps),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc, copy-scaled)",
+                     "alg_bytes_per_launch": alg_bytes / max(1, launches),
+                     "kernel": "k_fit_frames (persistent frame-per-workgroup doFit, %s)" % precision,
+                     "lm_ms": lm_ms, "lm_launches": launches,
+                     "avg_launch_us": (lm_ms * 1e3 / launches) if launches else None,
+                     "lm_cycles": cycles, "bytes_per_cycle": N_ROI_18 * ALG_BYTES_PER_PIXEL},
+        "scheduler": {"sweeps": st["sweeps"], "escape_reruns": st["escape_reruns"]},
+    }, (out, status, per_frame)
+
+
+def agreement(a, b, D):
+    """Flag agreement and, over peaks CONVERGED in both, the |dx|,|dy| and |dz| distributions."""
+    out_a, st_a = a
+    out_b, st_b = b
+    agree = float(numpy.mean(st_a == st_b)) if st_a.size else 1.0
+    both = (st_a == D.CONVERGED) & (st_b == D.CONVERGED)
+    dxy = numpy.maximum(numpy.abs(out_a[both, 0] - out_b[both, 0]), numpy.abs(out_a[both, 1] - out_b[both, 1]))
+    dz = numpy.abs(out_a[both, 2] - out_b[both, 2])
+    q = lambda v, t: float(numpy.quantile(v, t)) if v.size else 0.0  # noqa: E731
+    return {"flag_agreement": agree, "converged_a": int((st_a == D.CONVERGED).sum()),
+            "converged_b": int((st_b == D.CONVERGED).sum()),
+            "frac_dxy_within_1nm": float(numpy.mean(dxy < TOL_MIXED_XY_PX)) if dxy.size else 1.0,
+            "dxy_px_p50": q(dxy, 0.5), "dxy_px_p99": q(dxy, 0.99), "dxy_px_p999": q(dxy, 0.999),
+            "max_dxy_px": float(dxy.max(initial=0.0)),
+            "dz_um_p99": q(dz, 0.99), "max_dz_um": float(dz.max(initial=0.0))}
+
+
+def result_rows(outs):
+    rows = numpy.concatenate([numpy.stack([o["x"], o["y"], o["z"]], axis=1) for o in outs])
+    st = numpy.concatenate([o["status"] for o in outs])
+    return rows, st
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the fitter has no multi-GPU path (frames would be "
+              "sharded across GPUs with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
+    import torch
+    __graft_entry__.load_package()
+    from storm_analysis_amd.sa_library import dao_fit_c as D
+    from storm_analysis_amd.diagnostics import synth_gpu
+
+    dev = "cuda:0"
+    F, S, N = args.frames, args.size, args.emitters
+    t_gen = time.perf_counter()
+    log("generating %d frames of %dx%d with %d emitters on %s" % (F, S, S, N, dev))
+    images, backgrounds, rows, counts, truth = synth_gpu.frames_z(F, size=S, n=N, device=dev)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    counts_np = counts.numpy()
+
+    precisions = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    meas, res = {}, {}
+    for p in precisions:
+        meas[p], res[p] = run_batched(p, args, images, backgrounds, rows, counts_np)
+    head = precisions[0]
+
+    # plain path: one frame at a time through the single-frame C-ABI (FP64)
     host_images = images[:args.plain_frames].cpu().numpy()
     host_bgs = backgrounds[:args.plain_frames].cpu().numpy()
     host_rows = rows.cpu().numpy()
     fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
-    fit_plain(fac, host_images[0], host_bgs[0], frame_rows(host_rows, counts_np, 0))  # warm
-    lat = []
-    plain_conv = 0
-    plain_out = []
+    lat, plain_out, plain_conv = [], [], 0
+    if args.plain_frames > 0:
+        fit_plain(fac, host_images[0], host_bgs[0], frame_rows(host_rows, counts_np, 0))  # warm
     t1 = time.perf_counter()
     for f in range(args.plain_frames):
         ta = time.perf_counter()
@@ -220,79 +277,66 @@ def main():
         plain_conv += int((o["status"] == D.CONVERGED).sum())
         plain_out.append(o)
     plain_dt = time.perf_counter() - t1
-    plain_fps = plain_conv / plain_dt
-    lat_ms = numpy.array(lat) * 1e3
-
-    # batched and plain agree frame by frame
-    c = numpy.concatenate([[0], numpy.cumsum(per_frame)])
-    max_dxy = 0.0
-    flags_equal = True
-    for f in range(args.plain_frames):
-        b = out[c[f]:c[f + 1]]
-        st = status[c[f]:c[f + 1]]
-        flags_equal &= bool(numpy.array_equal(st, plain_out[f]["status"]))
-        ok = st != D.ERROR
-        if ok.any():
-            max_dxy = max(max_dxy, float(numpy.abs(b[ok, 0] - plain_out[f]["x"][ok]).max()),
-                          float(numpy.abs(b[ok, 1] - plain_out[f]["y"][ok]).max()))
-
+    plain_fps = plain_conv / plain_dt if plain_dt > 0 else 0.0
+    lat_ms = numpy.array(lat) * 1e3 if lat else numpy.zeros(1)
     log("plain: %d frames, p99 %.2f ms" % (args.plain_frames, numpy.percentile(lat_ms, 99)))
+
+    # accuracy: batched FP64 against plain (same arithmetic), mixed against FP64
+    accuracy = {}
+    if "fp64" in res:
+        out64, st64, per_frame = res["fp64"]
+        n_plain = int(numpy.sum(per_frame[:args.plain_frames]))
+        if n_plain:
+            prow, pst = result_rows(plain_out)
+            accuracy["fp64_batched_vs_plain"] = agreement((out64[:n_plain], st64[:n_plain]), (prow, pst), D)
+        if "mixed" in res:
+            outm, stm, _ = res["mixed"]
+            accuracy["mixed_vs_fp64_all_frames"] = agreement((outm, stm), (out64, st64), D)
+
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline on %d frames" % min(args.cpu_frames, F))
         nfr = min(args.cpu_frames, F)
-        host_images_cpu = images[:nfr].cpu().numpy()
-        host_bgs_cpu = backgrounds[:nfr].cpu().numpy()
-        cpu, cpu_out = cpu_baseline(host_images_cpu, host_bgs_cpu, host_rows, counts_np, nfr)
-        # GPU batched vs the oracle on the same frames
-        max_d = 0.0
-        for f in range(nfr):
-            b = out[c[f]:c[f + 1]]
-            st = status[c[f]:c[f + 1]]
-            flags_equal &= bool(numpy.array_equal(st, cpu_out[f]["status"]))
-            ok = st != D.ERROR
-            if ok.any():
-                max_d = max(max_d, float(numpy.abs(b[ok, 0] - cpu_out[f]["x"][ok]).max()),
-                            float(numpy.abs(b[ok, 1] - cpu_out[f]["y"][ok]).max()))
-        cpu["max_dxy_vs_gpu_px"] = max_d
+        log("cpu baseline (oracle) on %d frames" % nfr)
+        cpu, cpu_out = cpu_baseline(images[:nfr].cpu().numpy(), backgrounds[:nfr].cpu().numpy(), host_rows,
+                                    counts_np, nfr)
+        orow, ost = result_rows(cpu_out)
+        for p in precisions:
+            out_p, st_p, per_frame = res[p]
+            n = int(numpy.sum(per_frame[:nfr]))
+            accuracy["%s_vs_oracle" % p] = agreement((out_p[:n], st_p[:n]), (orow, ost), D)
 
-    st = bf.stats()
+    hm = meas[head]
     line = {
         "metric": "converged emitter fits/sec + p99 frame latency; batched vs plain, 1 MI355X",
-        "value": batched_fps,
+        "value": hm["fits_per_s"],
         "unit": "converged fits/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3 / args.steps,
+        "ms_per_step": hm["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "mixed: fp32 deviance/J/H terms, fp64 shape/accumulators/solve/chi2" if head == "mixed" else "f64",
         "data": "synthetic (seeded GPU generator: Poisson frames, offset 100 ADU, gain 1)",
         "config": {"workload": "3D astigmatic Z model, %d frames of %dx%d, %d emitters/frame, ROI %d, MLE" %
                                (F, S, S, N, ROI),
-                   "frames": F, "frame_size": S, "emitters_per_frame": N, "model": "Z", "roi_size": ROI},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_unit": "bytes per launch (rocprofv3 --pmc, copy-scaled)",
-                     "alg_bytes_per_launch": alg_bytes / max(1, lm_launches),
-                     "kernel": "k_fit_frames (persistent frame-per-workgroup doFit)", "lm_ms": lm_ms, "lm_launches": lm_launches,
-                     "avg_launch_us": (lm_ms * 1e3 / lm_launches) if lm_launches else None,
-                     "lm_cycles": cycles, "bytes_per_cycle": N_ROI_18 * ALG_BYTES_PER_PIXEL},
+                   "frames": F, "frame_size": S, "emitters_per_frame": N, "model": "Z", "roi_size": ROI,
+                   "precision": head},
+        "roofline": hm["roofline"],
         "cpu_baseline": cpu,
-        "plain": {"fits_per_s": plain_fps, "frames": args.plain_frames, "p50_ms": float(numpy.percentile(lat_ms, 50)),
-                  "p99_ms": float(numpy.percentile(lat_ms, 99)), "path": "MultiFitterZ, one frame per call"},
-        "batched_vs_plain": batched_fps / plain_fps if plain_fps > 0 else None,
+        "batched": meas,
+        "plain": {"fits_per_s": plain_fps, "frames": args.plain_frames,
+                  "p50_ms": float(numpy.percentile(lat_ms, 50)), "p99_ms": float(numpy.percentile(lat_ms, 99)),
+                  "path": "MultiFitterZ, one frame per call, FP64"},
+        "batched_vs_plain": hm["fits_per_s"] / plain_fps if plain_fps > 0 else None,
         "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
-        "parity": {"flags_equal_batched_plain_oracle": flags_equal, "max_dxy_batched_vs_plain_px": max_dxy},
-        "converged_per_step": conv // max(1, args.steps),
-        "scheduler": {"sweeps": st["sweeps"], "rounds": st["rounds"], "escape_reruns": st["escape_reruns"],
-                      "max_attempts": st["max_attempts"]},
+        "accuracy": accuracy,
+        "mixed_xy_bar_px": TOL_MIXED_XY_PX,
+        "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }
     print(json.dumps(line))
-    bf.cleanup()
 
 
 if __name__ == "__main__":

@@ -41,6 +41,11 @@ typedef struct daoBatchModel {
   double wy[5];
   double z_min;
   double z_max;
+  int precision;    /* 0: FP64 throughout (the reference's arithmetic);
+                       1: mixed -- FP32 per-pixel model, deviance terms,
+                       Jacobian and Hessian, FP64 fit-image accumulators,
+                       deviance sums, Cholesky solve and accept/reject
+                       (MLE only; other error models run in FP64) */
 } daoBatchModel;
 
 /* Create a batch of n_frames frames; peak_caps[f] bounds the peaks of frame f.

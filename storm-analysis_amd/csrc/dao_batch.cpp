@@ -54,6 +54,7 @@ daoBatch* daoBatchCreate(const daoBatchModel* bm, int n_frames, const int* width
   m.roi = bm->roi_size;
   m.jac = bm->shape == dao::SH_2DFIXED ? 4 : (bm->shape == dao::SH_3D ? 6 : 5);
   m.margin = bm->margin > 0 ? bm->margin : 2;
+  m.prec = bm->precision == dao::PREC_MIXED ? dao::PREC_MIXED : dao::PREC_FP64;
   m.xoff = 0.5 * (double)bm->roi_size - 1.0;
   m.zoff = 0.0;
   m.tol = bm->tolerance;

@@ -31,6 +31,9 @@ enum Counter {
 enum OpKind { OP_LM = 0, OP_RECENTER = 1, OP_NEWPEAK = 2, OP_SUBTRACT = 3, OP_PEAKERROR = 4 };
 // selection predicates over peak slots (k_flag)
 enum SelectMode { SEL_RUNNING = 0, SEL_RECENTER = 1, SEL_RUNNING_BAD = 2 };
+// arithmetic of the LM turns in k_fit_frames: FP64 throughout, or FP32
+// per-pixel work with FP64 accumulators, solve and accept/reject (MLE only)
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
 // start-peak formats of daoNewPeaks (dao_fit.c:1605, 1695)
 enum PeakFmt { PF_FINDER = 0, PF_TESTING = 1, PF_TEXT = 2 };
 
@@ -41,6 +44,7 @@ struct Model {
   int nroi;      // roi_n_index
   int jac;       // jac_size
   int margin;    // reserved-box margin around the ROI, pixels
+  int prec;      // Precision
   double xoff;   // == yoff
   double zoff;
   double tol;

@@ -1,0 +1,14 @@
+// k_fit_frames instantiations in FP64 (the reference's arithmetic): 8 waves
+// per workgroup, 4 for the widest ROIs, whose LDS slices are larger.
+#include "dao_ops.h"
+
+namespace dao {
+
+void launch_fit_frames_fp64(const FitArgs& a, int nframes, hipStream_t st) {
+  DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, ({
+                   constexpr int NW = PPL <= 8 ? 8 : 4;
+                   k_fit_frames<PPL, SHP, NW, false><<<nframes, NW * 64, 0, st>>>(a);
+                 })));
+}
+
+}  // namespace dao

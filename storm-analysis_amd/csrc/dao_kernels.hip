@@ -1,928 +1,14 @@
-// HIP kernels of the MI355X 3D-DAOSTORM fitter (gfx950).
-//
-// The reference visits peaks in index order and each peak read-modify-writes
-// the fit image over its ROI (storm_analysis/sa_library/multi_fit.c,
-// mFitIterateLM:1072-1349).  Here every image operation ("op": one LM turn,
-// one recentering, one new peak, ...) reserves a box of pixels, and op j
-// starts only after every earlier op i < j of its frame whose box overlaps;
-// all other ops run concurrently.  Each op is one 64-lane wavefront: pixel j
-// of the ROI lives in lane j % 64, slot j / 64, its fit values in registers
-// and its read-only data (x, rqe, scmos) in the wave's LDS slice, so a turn
-// reads and writes HBM once.  Sums over the ROI are wavefront shuffles; the
-// 4..6 parameter damped normal equations are solved in FP64 registers by
-// every lane (uniform control flow).
-//
-// Two drivers run ops:
-//  - k_fit_frames: one persistent workgroup per frame runs whole sweeps
-//    (mFitIterateLM, or all of doFit) without returning to the host;
-//  - the multi-kernel executor (k_bin_*, k_preds, k_round): one launch per
-//    round, for image operations outside the sweep loop (new peaks,
-//    subtraction) and as an alternative sweep driver.
-// An LM turn may move its ROI (multi_fit.c:1933-1946).  If a candidate ROI
-// leaves the reserved box the turn puts the frame back as it found it; the
-// driver then re-runs it in order (see fpw_run and Engine::sweep).
-#include <hip/hip_runtime.h>
-#include <math.h>
-#include <stdint.h>
-
-#include "dao_device.h"
+// Non-template kernels of the MI355X fitter and the host-side launchers
+// (the device code they share is in dao_ops.h; the k_fit_frames
+// instantiations are compiled in dao_fit_fp64.hip and dao_fit_mixed.hip).
+#include "dao_ops.h"
 
 namespace dao {
-
-// ---------------------------------------------------------------------------
-// wavefront helpers
-// ---------------------------------------------------------------------------
-
-__device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
-
-// v of the lane selected by the DPP control CTRL (quad permutes and row
-// mirrors: every lane has a source inside its 16-lane row).
-template <int CTRL>
-__device__ inline double dpp_mov(double v) {
-  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ inline double lane_value(double v, int lane) {
-  int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-// Sum over the wave, the same bits in every lane: pairs, quads, 8- and
-// 16-lane rows by DPP (each step adds two equal-order partial sums, so every
-// lane of a row holds the identical row sum), then (r0 + r1) + (r2 + r3).
-__device__ inline double wsum(double v) {
-  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_mov<0x141>(v);  // row_half_mirror
-  v += dpp_mov<0x140>(v);  // row_mirror
-  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
-}
-
-// ---------------------------------------------------------------------------
-// per-peak uniform state (every lane holds the same copy)
-// ---------------------------------------------------------------------------
-
-struct PeakState {
-  double prm[kNFit];
-  double lam;
-  double err;
-  int xi, yi;
-  int status;
-  int iter;
-  int added;
-};
-
-__device__ inline void load_state(const PeakRec* r, PeakState& s) {
-#pragma unroll
-  for (int q = 0; q < kNFit; q++) s.prm[q] = r->prm[q];
-  s.lam = r->lam;
-  s.err = r->err;
-  s.xi = r->xi;
-  s.yi = r->yi;
-  s.status = r->status;
-  s.iter = r->iter;
-  s.added = r->added;
-}
-
-__device__ inline void store_state(PeakRec* r, const PeakState& s) {
-  if (lane_id() == 0) {
-#pragma unroll
-    for (int q = 0; q < kNFit; q++) r->prm[q] = s.prm[q];
-    r->lam = s.lam;
-    r->err = s.err;
-    r->xi = s.xi;
-    r->yi = s.yi;
-    r->status = s.status;
-    r->iter = s.iter;
-    r->added = s.added;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Gaussian model (dao_fit.c)
-// ---------------------------------------------------------------------------
-
-// daoCalcWidthsFromZ (dao_fit.c:1009-1035); wxz[0] holds wo^2.
-__device__ inline void widths_from_z(const Model& m, PeakState& s) {
-  double u = (s.prm[P_ZC] - m.wxz[1]) / m.wxz[2];
-  double u2 = u * u;
-  double t = 1.0 + u2 + m.wxz[3] * u2 * u + m.wxz[4] * u2 * u2;
-  s.prm[P_XW] = 2.0 / (m.wxz[0] * t);
-  u = (s.prm[P_ZC] - m.wyz[1]) / m.wyz[2];
-  u2 = u * u;
-  t = 1.0 + u2 + m.wyz[3] * u2 * u + m.wyz[4] * u2 * u2;
-  s.prm[P_YW] = 2.0 / (m.wyz[0] * t);
-}
-
-// Chain factors d(width)/dz of daoCalcJHZ (dao_fit.c:904-916), without the
-// 1/d factor, as there.
-__device__ inline void z_grad(const Model& m, const PeakState& s, double& gx, double& gy) {
-  double u = (s.prm[P_ZC] - m.wxz[1]) / m.wxz[2];
-  double u2 = u * u;
-  double t = 1.0 + u2 + m.wxz[3] * u2 * u + m.wxz[4] * u2 * u2;
-  double zt = 2.0 * u + 3.0 * m.wxz[3] * u2 + 4.0 * m.wxz[4] * u2 * u;
-  gx = -2.0 * zt / (m.wxz[0] * (t * t));
-  u = (s.prm[P_ZC] - m.wyz[1]) / m.wyz[2];
-  u2 = u * u;
-  t = 1.0 + u2 + m.wyz[3] * u2 * u + m.wyz[4] * u2 * u2;
-  zt = 2.0 * u + 3.0 * m.wyz[3] * u2 + 4.0 * m.wyz[4] * u2 * u;
-  gy = -2.0 * zt / (m.wyz[0] * (t * t));
-}
-
-// One separable shape sample per lane: lanes 0..31 hold column offsets
-// xt = i - (X - xi) - xoff and ext = exp(-xt^2 XW); lanes 32..63 the rows
-// (dao_fit.c:968-998; the row offset also uses xoff, dao_fit.c:979).
-__device__ inline void lane_shape(const Model& m, const PeakState& s, double& t, double& e) {
-  int l = lane_id();
-  int i = l & 31;
-  bool row = l >= 32;
-  double o = row ? -(s.prm[P_YC] - (double)s.yi) - m.xoff : -(s.prm[P_XC] - (double)s.xi) - m.xoff;
-  double w = row ? s.prm[P_YW] : s.prm[P_XW];
-  t = (double)i + o;
-  e = exp(-t * t * w);
-}
-
-// ---------------------------------------------------------------------------
-// ROI cache of one wave
-// ---------------------------------------------------------------------------
-
-// The wave's LDS slice (RoiLds<PPL>::kDoubles doubles): read-only data of the
-// ROI pixels -- for slot s of lane l, x at [s*64+l], rqe at [(PPL+s)*64+l],
-// scmos at [(2PPL+s)*64+l], each lane touching only its own entries -- then
-// the shape table (64 (offset, exp) pairs, written by lane_shape) and the
-// reduced J and packed H of the turn (uniform values).
-template <int PPL>
-struct RoiLds {
-  static constexpr int kTab = 3 * PPL * kWave;
-  static constexpr int kHJ = kTab + 2 * kWave;
-  static constexpr int kDoubles = kHJ + 32;
-};
-
-template <int PPL>
-struct Roi {
-  int rxy[PPL];   // column | row << 8
-  int koff[PPL];  // row * W + column
-  double f[PPL], bg[PPL], tfi[PPL], psf[PPL];
-  int cnt[PPL];
-  unsigned stale;  // bit s: slot s stale
-  unsigned okm;    // bit s: slot s is a pixel of the ROI
-  long long base;  // pixel index of the ROI origin held
-  int at_x, at_y;
-  bool valid;
-  double* lds;
-
-  __device__ bool ok(int s) const { return (okm >> s) & 1u; }
-  __device__ int rx(int s) const { return rxy[s] & 0xff; }
-  __device__ int ry(int s) const { return rxy[s] >> 8; }
-  __device__ long long k(int s) const { return base + koff[s]; }
-  __device__ double& x(int s) { return lds[s * kWave + lane_id()]; }
-  __device__ double& rqe(int s) { return lds[(PPL + s) * kWave + lane_id()]; }
-  __device__ double& scm(int s) { return lds[(2 * PPL + s) * kWave + lane_id()]; }
-
-  __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
-    int l = lane_id();
-    okm = 0;
-#pragma unroll
-    for (int s = 0; s < PPL; s++) {
-      int j = l + s * kWave;
-      bool o = j < m.nroi;
-      int v = o ? roi_tab[j] : 0;
-      int cx = v & 0xffff, cy = v >> 16;
-      rxy[s] = cx | (cy << 8);
-      koff[s] = cy * fr.W + cx;
-      if (o) okm |= 1u << s;
-    }
-    lds = slice;
-    valid = false;
-  }
-
-  __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
-    base = fr.pix0 + (long long)yi * fr.W + xi;
-    stale = 0;
-#pragma unroll
-    for (int s = 0; s < PPL; s++) {
-      if (ok(s)) {
-        const PixRW w = px.rw[k(s)];
-        const PixRO r = px.ro[k(s)];
-        f[s] = w.f;
-        bg[s] = w.bg;
-        tfi[s] = w.tfi;
-        cnt[s] = w.cnt;
-        if (w.stale) stale |= 1u << s;
-        x(s) = r.x;
-        rqe(s) = r.rqe;
-        scm(s) = r.scm;
-      }
-    }
-    at_x = xi;
-    at_y = yi;
-    valid = true;
-  }
-
-  __device__ void store(const Pix& px) {
-    if (!valid) return;
-#pragma unroll
-    for (int s = 0; s < PPL; s++) {
-      if (ok(s)) {
-        PixRW w;
-        w.f = f[s];
-        w.bg = bg[s];
-        w.tfi = tfi[s];
-        w.cnt = cnt[s];
-        w.stale = (stale >> s) & 1u;
-        px.rw[k(s)] = w;
-      }
-    }
-  }
-
-  __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi) {
-    if (valid && at_x == xi && at_y == yi) return;
-    store(px);
-    load(px, fr, xi, yi);
-  }
-
-  __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLds<PPL>::kTab); }
-  __device__ double* hj() { return lds + RoiLds<PPL>::kHJ; }
-
-  // Shape table of state st in LDS: entry c < 32 = column c, 32 + r = row r.
-  __device__ void table(const Model& m, const PeakState& st) {
-    double t, e;
-    lane_shape(m, st, t, e);
-    __builtin_amdgcn_wave_barrier();
-    tab()[lane_id()] = make_double2(t, e);
-    __builtin_amdgcn_wave_barrier();
-  }
-
-  // psf at the held pixels for state st (cache at st.xi, st.yi); leaves the
-  // shape table of st in LDS.
-  __device__ void shape(const Model& m, const PeakState& st) {
-    table(m, st);
-    const double2* tb = tab();
-#pragma unroll
-    for (int s = 0; s < PPL; s++) psf[s] = tb[32 + ry(s)].y * tb[rx(s)].y;
-  }
-
-  // mFitAddPeak / mFitSubtractPeak (multi_fit.c:28-57, 1894-1922).
-  __device__ void stamp(const PeakState& st, int sign) {
-    double h = st.prm[P_HEIGHT];
-    double b = st.prm[P_BG];
-#pragma unroll
-    for (int s = 0; s < PPL; s++) {
-      if (ok(s)) {
-        double r = rqe(s);
-        double sc = scm(s);
-        if (sign > 0) {
-          f[s] += h * psf[s] * r;
-          cnt[s] += 1;
-          bg[s] += b * r + sc;
-        } else {
-          f[s] -= h * psf[s] * r;
-          cnt[s] -= 1;
-          bg[s] -= (b * r + sc);
-        }
-      }
-    }
-    stale = okm;
-  }
-};
-
-__device__ inline double anscombe(double v) {
-  if (v < (-0.375 + 1.0e-6)) return 0.0;
-  return 2.0 * sqrt(v + 0.375);
-}
-
-// mFitCalcErr{,ALS,LS,DWLS,FWLS} (multi_fit.c:111-458) over the held ROI.
-// Returns 1 when MLE/FWLS meet fi <= 0: pixels before the first such pixel
-// (ROI order) are refreshed, the rest keep their cached fi, and the error is
-// not updated.  Adds n_neg_fi into neg_fi.
-template <int PPL>
-__device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double& err, int& neg_fi) {
-  const int l = lane_id();
-  const bool stop_on_neg = (m.errm == EM_MLE || m.errm == EM_FWLS);
-  double sum = 0.0;
-  bool stopped = false;  // uniform: a slot held the ROI's first fi <= 0
-#pragma unroll
-  for (int s = 0; s < PPL; s++) {
-    if (!stopped) {
-      const bool o = c.ok(s);
-      double t = o ? c.f[s] + c.bg[s] / ((double)c.cnt[s]) : 1.0;
-      const unsigned long long bad = __ballot(o && stop_on_neg && t <= 0.0);
-      // pixels of this slot before the first bad one (ROI order j = s*64 + l)
-      const int lim = bad ? __builtin_ctzll(bad) : kWave;
-      bool zero = false;
-      if (o && l < lim) {
-        double xk = c.x(s);
-        double e;
-        if (m.errm == EM_MLE) {
-          e = 2.0 * ((t - xk) - xk * log(t / xk));
-        } else if (m.errm == EM_ALS) {
-          double a = anscombe(t);
-          zero = (a == 0.0) && ((c.stale >> s) & 1u);
-          t = a;
-          double d = a - asx[c.k(s)];
-          e = d * d;
-        } else if (m.errm == EM_LS) {
-          double d = t - xk;
-          e = d * d;
-        } else if (m.errm == EM_DWLS) {
-          double d = t - xk;
-          e = d * d / xk;
-        } else {
-          double d = t - xk;
-          e = d * d / t;
-        }
-        c.tfi[s] = t;
-        c.stale &= ~(1u << s);
-        sum += e;
-      }
-      if (m.errm == EM_ALS) neg_fi += __popcll(__ballot(zero));
-      if (bad) stopped = true;
-    }
-  }
-  if (stopped) {
-    neg_fi += 1;
-    return 1;
-  }
-  err = wsum(sum);
-  return 0;
-}
-
-// daoCalcJH* (dao_fit.c:48-958): J and the upper triangle of H, packed row by
-// row, reduced over the wave into the slice's J/H area (J first).  Reads the
-// cached transformed fit tfi and the shape table, which must hold st's.
-template <int PPL, int SHAPE>
-__device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const double* asx) {
-  constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
-  double a1 = st.prm[P_HEIGHT];
-  double wx = st.prm[P_XW];
-  double wy = (SHAPE == SH_3D || SHAPE == SH_Z) ? st.prm[P_YW] : wx;  // 2D/2DFixed: XWIDTH twice
-  double gx = 0.0, gy = 0.0;
-  if (SHAPE == SH_Z) z_grad(m, st, gx, gy);
-  const double2* tb = c.tab();
-
-  double j_acc[N];
-  double h_acc[N * (N + 1) / 2];
-#pragma unroll
-  for (int a = 0; a < N; a++) j_acc[a] = 0.0;
-#pragma unroll
-  for (int a = 0; a < N * (N + 1) / 2; a++) h_acc[a] = 0.0;
-
-#pragma unroll
-  for (int s = 0; s < PPL; s++) {
-    if (c.ok(s)) {
-      const double2 cx = tb[c.rx(s)];
-      const double2 cy = tb[32 + c.ry(s)];
-      const double xt = cx.x, ex = cx.y, yt = cy.x, ey = cy.y;
-      double fi = c.tfi[s];
-      double xk = c.x(s);
-      double rq = c.rqe(s);
-      double et = ex * ey;
-      double g[N];
-      if (SHAPE == SH_2DFIXED) {
-        g[0] = et;
-        g[1] = 2.0 * a1 * wx * xt * et;
-        g[2] = 2.0 * a1 * wy * yt * et;
-        g[3] = 1.0;
-      } else if (SHAPE == SH_2D) {
-        g[0] = et;
-        g[1] = 2.0 * a1 * wx * xt * et;
-        g[2] = 2.0 * a1 * wy * yt * et;
-        g[3] = -a1 * xt * xt * et - a1 * yt * yt * et;
-        g[4] = 1.0;
-      } else if (SHAPE == SH_3D) {
-        g[0] = et;
-        g[1] = 2.0 * a1 * wx * xt * et;
-        g[2] = -a1 * xt * xt * et;
-        g[3] = 2.0 * a1 * wy * yt * et;
-        g[4] = -a1 * yt * yt * et;
-        g[5] = 1.0;
-      } else {
-        g[0] = et;
-        g[1] = 2.0 * a1 * wx * xt * et;
-        g[2] = 2.0 * a1 * wy * yt * et;
-        g[3] = -a1 * xt * xt * gx * et - a1 * yt * yt * gy * et;
-        g[4] = 1.0;
-      }
-      const double rf = 1.0 / fi;
-      double sc = (m.errm == EM_ALS) ? rq * 2.0 * rf : rq;
-#pragma unroll
-      for (int a = 0; a < N; a++) g[a] *= sc;
-      double wj, wh;
-      if (m.errm == EM_MLE) {
-        const double xr = xk * rf;
-        wj = 2.0 * (1.0 - xr);
-        wh = 2.0 * xr * rf;
-      } else if (m.errm == EM_ALS) {
-        wj = fi - asx[c.k(s)];
-        wh = 1.0;
-      } else if (m.errm == EM_LS) {
-        wj = fi - xk;
-        wh = 1.0;
-      } else if (m.errm == EM_DWLS) {
-        const double rx = 1.0 / xk;
-        wj = (fi - xk) * rx;
-        wh = rx;
-      } else {
-        double t1 = (fi - xk) * rf;
-        wj = 2.0 * t1 - t1 * t1;
-        wh = 2.0 * rf * (xk * rf - t1 + t1 * t1);
-      }
-      int q = 0;
-#pragma unroll
-      for (int a = 0; a < N; a++) {
-        j_acc[a] += wj * g[a];
-#pragma unroll
-        for (int b = a; b < N; b++) h_acc[q++] += wh * g[a] * g[b];
-      }
-    }
-  }
-  double* out = c.hj();
-#pragma unroll
-  for (int a = 0; a < N; a++) {
-    const double v = wsum(j_acc[a]);
-    if (lane_id() == 0) out[a] = v;
-  }
-#pragma unroll
-  for (int a = 0; a < N * (N + 1) / 2; a++) {
-    const double v = wsum(h_acc[a]);
-    if (lane_id() == 0) out[N + a] = v;
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-template <int N>
-__device__ constexpr int upk(int r, int c) {  // packed index of (r, c), r <= c
-  return r * N - r * (r - 1) / 2 + (c - r);
-}
-
-// mFitSolve (multi_fit.c:1839-1883): LAPACK dposv "Lower" on row-major
-// storage = Cholesky of the symmetric matrix given by the upper triangle,
-// whose diagonal is scaled by (1 + lambda) (multi_fit.c:1174-1185).
-// Non-zero on a pivot that is not > 0 (or NaN) or a NaN in the solution.
-template <int N>
-__device__ inline int chol_solve(const double* H, double lam, double* b) {
-  double L[N][N];
-  double inv[N];  // 1 / L[j][j]
-#pragma unroll
-  for (int j = 0; j < N; j++) {
-    double s = (1.0 + lam) * H[upk<N>(j, j)];
-#pragma unroll
-    for (int k = 0; k < j; k++) s -= L[j][k] * L[j][k];
-    if (!(s > 0.0)) return j + 1;
-    const double d = sqrt(s);
-    inv[j] = 1.0 / d;
-#pragma unroll
-    for (int i = j + 1; i < N; i++) {
-      double t = H[upk<N>(j, i)];
-#pragma unroll
-      for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
-      L[i][j] = t * inv[j];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < N; i++) {
-    double t = b[i];
-#pragma unroll
-    for (int k = 0; k < i; k++) t -= L[i][k] * b[k];
-    b[i] = t * inv[i];
-  }
-#pragma unroll
-  for (int i = N - 1; i >= 0; i--) {
-    double t = b[i];
-#pragma unroll
-    for (int k = i + 1; k < N; k++) t -= L[k][i] * b[k];
-    b[i] = t * inv[i];
-  }
-#pragma unroll
-  for (int i = 0; i < N; i++)
-    if (isnan(b[i])) return 1;
-  return 0;
-}
-
-// mFitUpdate (multi_fit.c:1933-1946).
-__device__ inline void roi_follow(PeakState& s) {
-  double dx = s.prm[P_XC] - s.xi;
-  if ((dx < -0.5) || (dx > 1.5)) s.xi = (int)floor(s.prm[P_XC]);
-  double dy = s.prm[P_YC] - s.yi;
-  if ((dy < -0.5) || (dy > 1.5)) s.yi = (int)floor(s.prm[P_YC]);
-}
-
-// daoUpdate2DFixed/2D/3D/Z (dao_fit.c:1777-1880).
-template <int SHAPE>
-__device__ inline void apply_update(const Model& m, PeakState& s, const double* d) {
-  if (SHAPE == SH_2DFIXED) {
-    s.prm[P_HEIGHT] -= d[0];
-    s.prm[P_XC] -= d[1];
-    s.prm[P_YC] -= d[2];
-    s.prm[P_BG] -= d[3];
-    roi_follow(s);
-  } else if (SHAPE == SH_2D) {
-    s.prm[P_HEIGHT] -= d[0];
-    s.prm[P_XC] -= d[1];
-    s.prm[P_YC] -= d[2];
-    s.prm[P_XW] -= d[3];
-    s.prm[P_BG] -= d[4];
-    s.prm[P_YW] = s.prm[P_XW];
-    roi_follow(s);
-  } else if (SHAPE == SH_3D) {
-    s.prm[P_HEIGHT] -= d[0];
-    s.prm[P_XC] -= d[1];
-    s.prm[P_XW] -= d[2];
-    s.prm[P_YC] -= d[3];
-    s.prm[P_YW] -= d[4];
-    s.prm[P_BG] -= d[5];
-    roi_follow(s);
-  } else {
-    s.prm[P_HEIGHT] -= d[0];
-    s.prm[P_XC] -= d[1];
-    s.prm[P_YC] -= d[2];
-    s.prm[P_ZC] -= d[3];
-    s.prm[P_BG] -= d[4];
-    if (s.prm[P_ZC] < m.zmin) s.prm[P_ZC] = m.zmin;
-    if (s.prm[P_ZC] > m.zmax) s.prm[P_ZC] = m.zmax;
-    roi_follow(s);
-    widths_from_z(m, s);
-  }
-}
-
-// ROI origin inside the frame (every pixel of the ROI addressable).  The
-// reference never checks this before touching pixels; a peak whose origin is
-// off-image (e.g. an ERROR peak set back to RUNNING by the caller) would read
-// outside its arrays there, and is turned into an ERROR peak here instead.
-__device__ inline bool origin_ok(const Model& m, const Frame& fr, int xi, int yi) {
-  return xi >= 0 && yi >= 0 && xi + m.roi <= fr.W && yi + m.roi <= fr.H;
-}
-
-// mFitCheck (multi_fit.c:470-502).
-__device__ inline int margin_height_check(const Model& m, const Frame& fr, const PeakState& s, int* d) {
-  if ((s.xi < 0) || (s.xi >= (fr.W - m.roi)) || (s.yi < 0) || (s.yi >= (fr.H - m.roi))) {
-    d[C_MARGIN]++;
-    return 1;
-  }
-  if (s.prm[P_HEIGHT] < 0.0) {
-    d[C_NEG_HEIGHT]++;
-    return 1;
-  }
-  return 0;
-}
-
-// daoCheck2DFixed/2D/3D/Z (dao_fit.c:1047-1133): 2D and 3D clamp the widths.
-template <int SHAPE>
-__device__ inline int peak_check(const Model& m, const Frame& fr, PeakState& s, int* d) {
-  if (margin_height_check(m, fr, s, d)) return 1;
-  if (SHAPE == SH_2D || SHAPE == SH_3D) {
-    if (s.prm[P_XW] < m.wmin) s.prm[P_XW] = m.wmin;
-    else if (s.prm[P_XW] > m.wmax) s.prm[P_XW] = m.wmax;
-  }
-  if (SHAPE == SH_3D) {
-    if (s.prm[P_YW] < m.wmin) s.prm[P_YW] = m.wmin;
-    else if (s.prm[P_YW] > m.wmax) s.prm[P_YW] = m.wmax;
-  }
-  return 0;
-}
-
-// mFitResetPeak (multi_fit.c:1788-1798): back to the turn's start (still in
-// the peak's record), keeping `added`, lambda times 4.
-__device__ inline void reset_from(PeakState& w, const PeakRec* r) {
-  int added = w.added;
-  double lam = w.lam;
-  load_state(r, w);
-  w.added = added;
-  w.lam = lam * 4.0;
-}
-
-// ---------------------------------------------------------------------------
-// the operations
-// ---------------------------------------------------------------------------
-
-// One Levenberg-Marquardt turn of peak p: the body of the loop at
-// multi_fit.c:1089-1345.  The turn may touch only pixels inside `box`.
-// Returns 1 when the turn completed, 2 when a candidate ROI left the box: the
-// frame is then exactly as before the turn (the starting contribution is put
-// back) and the peak record is untouched.  d[] receives the turn's counter
-// increments.
-template <int PPL, int SHAPE>
-__device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                       double* slice, int p, int4 box, int* d) {
-  constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
-  PeakRec* rec = pk.rec + p;
-  // The record keeps the turn's starting state until the turn ends, so
-  // resets read it back from there instead of holding a second copy.
-  PeakState w;
-  load_state(rec, w);
-  if (!origin_ok(m, fr, w.xi, w.yi)) {
-    w.status = ST_ERROR;
-    store_state(rec, w);
-    return 1;
-  }
-
-  Roi<PPL> c;
-  c.init(m, roi_tab, fr, slice);
-  c.load(px, fr, w.xi, w.yi);
-  c.shape(m, w);
-
-  double wJ[N];
-  double e;
-  if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) w.err = e;
-  const double start = w.err;
-  calc_jh<PPL, SHAPE>(m, c, w, px.asx);
-  const double* J = c.hj();
-  const double* H = c.hj() + N;
-  c.stamp(w, -1);
-  w.added--;
-
-  bool escaped = false;
-  // Every retry multiplies lambda by 4 from >= 7.5e-4, so more than 60
-  // cycles cannot happen; the cap only guarantees termination.
-  for (int guard = 0;; guard++) {
-    if (w.lam > 1.0e20 || guard > 200) {
-      d[C_LOST]++;
-      w.status = ST_ERROR;
-      break;
-    }
-    w.iter++;
-    d[C_ITER]++;
-    w.status = ST_RUNNING;
-
-#pragma unroll
-    for (int r = 0; r < N; r++) wJ[r] = J[r];
-    if (chol_solve<N>(H, w.lam, wJ) != 0) {
-      d[C_DPOSV]++;
-      w.status = ST_ERROR;
-      w.lam = w.lam * 4.0;
-      continue;
-    }
-
-    apply_update<SHAPE>(m, w, wJ);
-    if (peak_check<SHAPE>(m, fr, w, d)) {
-      reset_from(w, rec);
-      w.status = ST_ERROR;
-      continue;
-    }
-
-    if (w.xi < box.x || w.yi < box.y || w.xi + m.roi > box.z || w.yi + m.roi > box.w) {
-      escaped = true;
-      break;
-    }
-
-    c.move_to(px, fr, w.xi, w.yi);
-    c.shape(m, w);
-    c.stamp(w, +1);
-    w.added++;
-
-    if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) != 0) {
-      c.stamp(w, -1);
-      w.added--;
-      reset_from(w, rec);
-      w.status = ST_ERROR;
-      continue;
-    }
-    w.err = e;
-
-    if (w.err > start) {
-      if (((w.err - start) / start) < m.tol) {
-        w.status = ST_CONVERGED;
-        break;
-      }
-      d[C_NON_DECR]++;
-      c.stamp(w, -1);
-      w.added--;
-      reset_from(w, rec);
-    } else {
-      if (((start - w.err) / start) < m.tol) {
-        w.status = ST_CONVERGED;
-        break;
-      }
-      if (w.lam > 1.0e-3) w.lam = 0.75 * w.lam;
-      break;
-    }
-  }
-
-  if (escaped) {
-    // Put the frame back: re-add the turn's starting contribution.  The
-    // escape is detected before the candidate touches any pixel.
-    load_state(rec, w);
-    c.move_to(px, fr, w.xi, w.yi);
-    c.shape(m, w);
-    c.stamp(w, +1);
-    c.store(px);
-#pragma unroll
-    for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-    return 2;
-  }
-
-  c.store(px);
-  store_state(rec, w);
-  return 1;
-}
-
-// mFitRecenterPeaks body (multi_fit.c:1656-1688) for peak p, whose centre
-// left [-0.25, 1.25] of its ROI origin.
-template <int PPL>
-__device__ void recenter_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr,
-                              const int* roi_tab, double* slice, int p, int* d) {
-  PeakRec* rec = pk.rec + p;
-  PeakState w;
-  load_state(rec, w);
-  if (!origin_ok(m, fr, w.xi, w.yi)) {
-    w.status = ST_ERROR;
-    store_state(rec, w);
-    return;
-  }
-  Roi<PPL> c;
-  c.init(m, roi_tab, fr, slice);
-  c.load(px, fr, w.xi, w.yi);
-  c.shape(m, w);
-  c.stamp(w, -1);
-  w.added--;
-  w.xi = (int)floor(w.prm[P_XC]);
-  w.yi = (int)floor(w.prm[P_YC]);
-  if (margin_height_check(m, fr, w, d)) {
-    d[C_LOST] += 1;
-    w.status = ST_ERROR;
-  } else {
-    c.move_to(px, fr, w.xi, w.yi);
-    c.shape(m, w);
-    c.stamp(w, +1);
-    w.added++;
-  }
-  c.store(px);
-  store_state(rec, w);
-}
-
-// Add (sign +1) or remove (-1) `added` copies of state s's contribution with
-// atomic adds (undos; the callers order overlapping ones).
-template <int PPL>
-__device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const Roi<PPL>& c,
-                           const PeakState& s, int sign) {
-  const int n = s.added * sign;
-  if (n == 0 || !origin_ok(m, fr, s.xi, s.yi)) return;
-  double t, e;
-  lane_shape(m, s, t, e);
-  const long long base = fr.pix0 + (long long)s.yi * fr.W + s.xi;
-#pragma unroll
-  for (int q = 0; q < PPL; q++) {
-    double ex = __shfl(e, c.rx(q), kWave);
-    double ey = __shfl(e, 32 + c.ry(q), kWave);
-    if (c.ok(q)) {
-      const long long k = base + c.koff[q];
-      const PixRO r = px.ro[k];
-      atomicAdd(&px.rw[k].f, (double)n * (s.prm[P_HEIGHT] * (ey * ex) * r.rqe));
-      atomicAdd(&px.rw[k].bg, (double)n * (s.prm[P_BG] * r.rqe + r.scm));
-      atomicAdd(&px.rw[k].cnt, n);
-      px.rw[k].stale = 1;
-    }
-  }
-}
-
-// Undo a completed LM turn of peak p: the image loses the peak's current
-// contribution and regains the one it had at the sweep's start (backup
-// record), and the peak returns to the backup.
-template <int PPL>
-__device__ void undo_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                          double* slice, int p) {
-  PeakState w, s0;
-  load_state(pk.rec + p, w);
-  load_state(pk.bk + p, s0);
-  Roi<PPL> c;
-  c.init(m, roi_tab, fr, slice);
-  undo_stamp<PPL>(m, px, fr, c, w, -1);
-  undo_stamp<PPL>(m, px, fr, c, s0, +1);
-  store_state(pk.rec + p, s0);
-}
-
-// daoNewPeaks body (dao_fit.c:1605-1751) for one new peak from a row of 4
-// ({x,y,z,sigma}) or 7 ({x,y,z,bg,h,sx,sy}) doubles.
-template <int PPL>
-__device__ void newpeak_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                             double* slice, int p, const double* q, int fmt, int row, int* d) {
-  PeakState w;
-#pragma unroll
-  for (int k = 0; k < kNFit; k++) w.prm[k] = 0.0;
-  w.added = 0;
-  w.iter = 0;
-  w.status = ST_RUNNING;
-  w.err = 0.0;
-  w.lam = 1.0;
-  w.prm[P_XC] = q[0] - m.xoff;
-  w.prm[P_YC] = q[1] - m.xoff;
-  w.prm[P_ZC] = q[2] - m.zoff;
-  if (fmt == PF_TEXT) {
-    w.prm[P_BG] = q[3];
-    w.prm[P_HEIGHT] = q[4];
-    w.prm[P_XW] = 1.0 / (2.0 * q[5] * q[5]);
-    w.prm[P_YW] = 1.0 / (2.0 * q[6] * q[6]);
-    if (m.shape == SH_Z) widths_from_z(m, w);
-  } else {
-    if (m.shape == SH_Z) {
-      widths_from_z(m, w);
-    } else {
-      double width = 1.0 / (2.0 * q[3] * q[3]);
-      w.prm[P_XW] = width;
-      w.prm[P_YW] = width;
-    }
-  }
-  w.xi = (int)floor(w.prm[P_XC]);
-  w.yi = (int)floor(w.prm[P_YC]);
-  if (fmt != PF_TEXT) {
-    w.prm[P_BG] = 1.0;
-    w.prm[P_HEIGHT] = 1.0;
-  }
-
-  int bad;
-  switch (m.shape) {
-    case SH_2D: bad = peak_check<SH_2D>(m, fr, w, d); break;
-    case SH_3D: bad = peak_check<SH_3D>(m, fr, w, d); break;
-    default: bad = peak_check<SH_2DFIXED>(m, fr, w, d); break;  // no width clamp
-  }
-  bool good = bad == 0;
-  Roi<PPL> c;
-  c.init(m, roi_tab, fr, slice);
-  if (good) {
-    c.load(px, fr, w.xi, w.yi);
-    c.shape(m, w);
-    if (fmt != PF_TEXT) {
-      // mFitEstimatePeakBackground (multi_fit.c:638-662)
-      double sb = 0.0;
-#pragma unroll
-      for (int s = 0; s < PPL; s++)
-        if (c.ok(s)) sb += px.ro[c.k(s)].bgest / c.rqe(s);
-      w.prm[P_BG] = wsum(sb) / ((double)m.nroi);
-      if (fmt == PF_FINDER) {
-        // mFitEstimatePeakHeight (multi_fit.c:678-700)
-        double sp = 0.0, sx = 0.0;
-#pragma unroll
-        for (int s = 0; s < PPL; s++) {
-          if (c.ok(s)) {
-            double t = c.psf[s] * c.rqe(s);
-            sp += t * t * t;
-            sx += t * t * (c.x(s) - c.f[s] - px.ro[c.k(s)].bgest);
-          }
-        }
-        w.prm[P_HEIGHT] = wsum(sx) / wsum(sp);
-        if (w.prm[P_HEIGHT] < m.min_height) w.prm[P_HEIGHT] = m.min_height;
-        if (w.prm[P_HEIGHT] <= 0.0) {
-          if (lane_id() == 0) printf("Warning peak %d has negative estimated height!\n", row);
-          good = false;
-        }
-      }
-    }
-  }
-  if (good) {
-    c.stamp(w, +1);
-    w.added++;
-    c.store(px);
-  } else {
-    w.status = ST_ERROR;
-  }
-  store_state(pk.rec + p, w);
-}
-
-// Initial error of a new peak (dao_fit.c:1758-1763).
-template <int PPL>
-__device__ void peakerror_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr,
-                               const int* roi_tab, double* slice, int p, int* d) {
-  PeakState w;
-  load_state(pk.rec + p, w);
-  if (w.status == ST_RUNNING && origin_ok(m, fr, w.xi, w.yi)) {
-    Roi<PPL> c;
-    c.init(m, roi_tab, fr, slice);
-    c.load(px, fr, w.xi, w.yi);
-    double e;
-    if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) w.err = e;
-    c.store(px);
-    store_state(pk.rec + p, w);
-  }
-}
-
-// mFitSetPeakStatus (multi_fit.c:1809-1831): subtract a peak marked ERROR.
-template <int PPL>
-__device__ void subtract_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr,
-                              const int* roi_tab, double* slice, int p) {
-  PeakState w;
-  load_state(pk.rec + p, w);
-  if (origin_ok(m, fr, w.xi, w.yi)) {
-    Roi<PPL> c;
-    c.init(m, roi_tab, fr, slice);
-    c.load(px, fr, w.xi, w.yi);
-    c.shape(m, w);
-    c.stamp(w, -1);
-    w.added--;
-    c.store(px);
-    store_state(pk.rec + p, w);
-  }
-}
 
 // ---------------------------------------------------------------------------
 // multi-kernel executor: predecessor lists and rounds
 // ---------------------------------------------------------------------------
 
-__device__ inline bool boxes_overlap(int4 a, int4 b) {
-  return (a.x < b.z) && (b.x < a.z) && (a.y < b.w) && (b.y < a.w);
-}
 
 __device__ inline bool is_big(const Sched& sc, int4 b) {
   return (b.z - b.x) > sc.cell || (b.w - b.y) > sc.cell;
@@ -1127,242 +213,6 @@ __global__ __launch_bounds__(256) void k_rollback(OpArgs a) {
   const int fi = a.pk.rec[p].frame;
   if (!a.sc.frame_bad[fi] || (a.sc.op_flags[i] & 1) == 0) return;
   undo_turn<PPL>(a.m, a.px, a.pk, a.frames[fi], a.roi_tab, lds[wv], p);
-}
-
-// ---------------------------------------------------------------------------
-// frame-per-workgroup fitter
-// ---------------------------------------------------------------------------
-//
-// One workgroup of NW waves runs whole sweeps of one frame (one
-// mFitIterateLM, or all of doFit, dao_fit_c.py:332-356) without returning to
-// the host.  The running peaks of a sweep are the ops 0..n-1 in index order.
-// Ops are handed out in index order, so every op below the one a wave just
-// took is finished or held by another wave; op j starts once no other wave
-// holds an op i < j whose box overlaps j's.  The wave with the smallest held
-// op can always start, so the frame makes progress, and overlapping ops run
-// in index order -- the reference's update order (multi_fit.c:1089).
-
-constexpr int kNoEscape = 0x7fffffff;
-constexpr int kIdle = 0x7fffffff;  // wave holds no op
-constexpr int kPending = -1;       // wave is taking an op (index not known yet)
-
-template <int NW>
-struct FpwShared {
-  int next;         // next op to hand out
-  int esc;          // smallest op whose LM turn left its box
-  int cur[NW];      // op held by each wave (kIdle / kPending / index)
-  int4 curbox[NW];  // its box
-  int wsum[NW];
-  int cnt[C_COUNT];
-};
-
-// Order-preserving list of the frame's peaks that satisfy `mode`
-// (SEL_RUNNING or SEL_RECENTER) into op_peak[peak0 + k]; returns the count.
-template <int NW>
-__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<NW>& sh) {
-  const int lane = lane_id(), wv = threadIdx.x >> 6;
-  int count = 0;  // every thread keeps the same running total
-  for (int base = 0; base < fr.nfit; base += NW * kWave) {
-    int t = base + threadIdx.x;
-    bool flag = false;
-    if (t < fr.nfit) {
-      const PeakRec* r = a.pk.rec + fr.peak0 + t;
-      if (r->status == ST_RUNNING) {
-        if (mode == SEL_RUNNING) {
-          flag = true;
-        } else {
-          double dx = r->prm[P_XC] - r->xi;
-          double dy = r->prm[P_YC] - r->yi;
-          flag = (dx < -0.25) || (dx > 1.25) || (dy < -0.25) || (dy > 1.25);
-        }
-      }
-    }
-    unsigned long long mask = __ballot(flag);
-    int pre = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) sh.wsum[wv] = __popcll(mask);
-    __syncthreads();
-    int off = count;
-    int tot = 0;
-    for (int u = 0; u < NW; u++) {
-      if (u < wv) off += sh.wsum[u];
-      tot += sh.wsum[u];
-    }
-    if (flag) a.op_peak[fr.peak0 + off + pre] = fr.peak0 + t;
-    count += tot;
-    __syncthreads();
-  }
-  return count;
-}
-
-// Wait until op j may start.  Returns false when an earlier op escaped (op j
-// must not run now) or the wait bound was hit.
-template <int NW>
-__device__ bool fpw_wait(const FitArgs& a, int j, int4 b, FpwShared<NW>& sh) {
-  const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (long long spin = 0;; spin++) {
-    bool blocked = false;
-    if (lane < NW && lane != wv) {
-      int c = __hip_atomic_load(&sh.cur[lane], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (c == kPending) {
-        blocked = true;
-      } else if (c < j) {
-        blocked = boxes_overlap(b, sh.curbox[lane]);
-      }
-    }
-    if (!__any(blocked)) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      return true;
-    }
-    if (__hip_atomic_load(&sh.esc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < j) return false;
-    if (spin > (1ll << 22)) {
-      if (lane == 0) atomicExch(a.err_flag, 1);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-}
-
-__device__ inline void fpw_set_cur(int* cur, int v) {
-  if (lane_id() == 0) __hip_atomic_store(cur, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Run ops 0..n-1 (kind OP_LM or OP_RECENTER) of the frame.  If an LM turn
-// leaves its box, every op below it has finished (ops are handed out in order
-// and never wait on later ones): the finished ops above it are undone newest
-// first, the turn gets the whole frame as its box (it then runs alone and
-// cannot escape again), and the run resumes from it.
-template <int PPL, int SHAPE, int NW>
-__device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwShared<NW>& sh, double* slice) {
-  const int lane = lane_id(), wv = threadIdx.x >> 6;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) a.op_done[fr.peak0 + k] = 0;
-  int from = 0;
-  for (;;) {
-    if (threadIdx.x < NW) sh.cur[threadIdx.x] = kIdle;
-    if (threadIdx.x == 0) {
-      sh.esc = kNoEscape;
-      sh.next = from;
-    }
-    __syncthreads();
-    for (;;) {
-      if (__hip_atomic_load(&sh.esc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != kNoEscape) break;
-      fpw_set_cur(&sh.cur[wv], kPending);
-      int j = 0;
-      if (lane == 0) j = atomicAdd(&sh.next, 1);
-      j = __shfl(j, 0, kWave);
-      if (j >= n) {
-        fpw_set_cur(&sh.cur[wv], kIdle);
-        break;
-      }
-      const int slot = fr.peak0 + j;
-      const int4 b = a.box[slot];
-      if (lane == 0) sh.curbox[wv] = b;
-      fpw_set_cur(&sh.cur[wv], j);
-      if (!fpw_wait<NW>(a, j, b, sh)) {
-        fpw_set_cur(&sh.cur[wv], kIdle);
-        break;
-      }
-      const int p = a.op_peak[slot];
-      int d[C_COUNT];
-#pragma unroll
-      for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-      int r = 1;
-      if (kind == OP_LM) {
-        r = lm_turn<PPL, SHAPE>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d);
-      } else {
-        recenter_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
-      }
-      if (r == 2) {
-        if (lane == 0) atomicMin(&sh.esc, j);
-        fpw_set_cur(&sh.cur[wv], kIdle);
-        break;
-      }
-      if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < C_COUNT; q++) {
-          a.op_delta[(long long)slot * C_COUNT + q] = d[q];
-          if (d[q]) atomicAdd(&sh.cnt[q], d[q]);
-        }
-        a.op_done[slot] = 1;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      fpw_set_cur(&sh.cur[wv], kIdle);
-    }
-    __syncthreads();
-    const int e = sh.esc;
-    if (e == kNoEscape) break;
-    if (wv == 0) {
-      for (int k = n - 1; k > e; k--) {
-        const int slot = fr.peak0 + k;
-        if (a.op_done[slot]) {
-          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, a.op_peak[slot]);
-          if (lane == 0) {
-            for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= a.op_delta[(long long)slot * C_COUNT + q];
-            a.op_done[slot] = 0;
-          }
-        }
-      }
-      if (lane == 0) a.box[fr.peak0 + e] = make_int4(0, 0, fr.W, fr.H);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    }
-    __syncthreads();
-    from = e;
-  }
-}
-
-template <int PPL, int SHAPE, int NW>
-__global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
-  __shared__ FpwShared<NW> sh;
-  __shared__ double lds[NW][RoiLds<PPL>::kDoubles];
-  const int wv = threadIdx.x >> 6;
-  const int f = a.frame_list[blockIdx.x];
-  const Frame fr = a.frames[f];
-  const Model& m = a.m;
-  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
-  __syncthreads();
-  int sweeps = 0;
-  for (int it = 0;; it++) {
-    const int n = fpw_list<NW>(a, fr, SEL_RUNNING, sh);
-    if (it > 0 && (n == 0 || it > a.max_more)) break;
-    if (n == 0) continue;  // the unconditional first sweep found nothing
-    sweeps++;
-    // reserved boxes (ROI at the sweep's start + margin) and backups
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      const int slot = fr.peak0 + k;
-      const int p = a.op_peak[slot];
-      const PeakRec r = a.pk.rec[p];
-      int4 b;
-      b.x = max(0, r.xi - m.margin);
-      b.y = max(0, r.yi - m.margin);
-      b.z = min(fr.W, r.xi + m.roi + m.margin);
-      b.w = min(fr.H, r.yi + m.roi + m.margin);
-      a.box[slot] = b;
-      a.pk.bk[p] = r;
-    }
-    __syncthreads();
-    fpw_run<PPL, SHAPE, NW>(a, fr, OP_LM, n, sh, lds[wv]);
-    // mFitRecenterPeaks (multi_fit.c:1644-1690)
-    const int nr = fpw_list<NW>(a, fr, SEL_RECENTER, sh);
-    if (nr > 0) {
-      for (int k = threadIdx.x; k < nr; k += blockDim.x) {
-        const int slot = fr.peak0 + k;
-        const PeakRec r = a.pk.rec[a.op_peak[slot]];
-        int nx = (int)floor(r.prm[P_XC]);
-        int ny = (int)floor(r.prm[P_YC]);
-        int4 b;
-        b.x = max(0, min(fr.W, min(r.xi, nx)));
-        b.y = max(0, min(fr.H, min(r.yi, ny)));
-        b.z = max(b.x, min(fr.W, max(r.xi, nx) + m.roi));
-        b.w = max(b.y, min(fr.H, max(r.yi, ny) + m.roi));
-        a.box[slot] = b;
-      }
-      __syncthreads();
-      fpw_run<PPL, SHAPE, NW>(a, fr, OP_RECENTER, nr, sh, lds[wv]);
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
-    atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
-  if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;
 }
 
 // ---------------------------------------------------------------------------
@@ -1704,36 +554,6 @@ __global__ void k_pack_peaks(Peaks pk, const Frame* frames, const int* offset, M
 // host-side launchers
 // ---------------------------------------------------------------------------
 
-static int ppl_for(int nroi) {
-  int p = (nroi + kWave - 1) / kWave;
-  if (p <= 2) return 2;
-  if (p <= 4) return 4;
-  if (p <= 6) return 6;
-  if (p <= 8) return 8;
-  if (p <= 12) return 12;
-  return 16;
-}
-
-#define DAO_PPL_SWITCH(nroi, KCALL)                    \
-  switch (ppl_for(nroi)) {                             \
-    case 2: { constexpr int PPL = 2; KCALL; } break;   \
-    case 4: { constexpr int PPL = 4; KCALL; } break;   \
-    case 6: { constexpr int PPL = 6; KCALL; } break;   \
-    case 8: { constexpr int PPL = 8; KCALL; } break;   \
-    case 12: { constexpr int PPL = 12; KCALL; } break; \
-    default: { constexpr int PPL = 16; KCALL; } break; \
-  }
-
-#define DAO_SHAPE_SWITCH(shape, KCALL)                                 \
-  switch (shape) {                                                     \
-    case SH_2DFIXED: { constexpr int SHP = SH_2DFIXED; KCALL; } break; \
-    case SH_2D: { constexpr int SHP = SH_2D; KCALL; } break;           \
-    case SH_3D: { constexpr int SHP = SH_3D; KCALL; } break;           \
-    default: { constexpr int SHP = SH_Z; KCALL; } break;               \
-  }
-
-static inline dim3 waves_grid(int n) { return dim3((n + kWavesPerBlock - 1) / kWavesPerBlock); }
-static inline dim3 flat_grid(long long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 void launch_op_round(int kind, const OpArgs& a, hipStream_t st, const double* rows, int fmt, int) {
   if (a.sc.nops == 0) return;
@@ -1840,14 +660,15 @@ void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, 
   k_pack_peaks<<<flat_grid(nslots), 256, 0, st>>>(pk, frames, offset, m, nslots, out, status);
 }
 
-// k_fit_frames: 8 waves per workgroup (4 for the widest ROIs, whose LDS
-// slices are larger).
+// k_fit_frames, FP64 or mixed precision (MLE only); the instantiations are
+// compiled in dao_fit_fp64.hip and dao_fit_mixed.hip.
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st) {
   if (nframes == 0) return;
-  DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, ({
-                   constexpr int NW = PPL <= 8 ? 8 : 4;
-                   k_fit_frames<PPL, SHP, NW><<<nframes, NW * 64, 0, st>>>(a);
-                 })));
+  if (a.m.prec == PREC_MIXED && a.m.errm == EM_MLE) {
+    launch_fit_frames_mixed(a, nframes, st);
+  } else {
+    launch_fit_frames_fp64(a, nframes, st);
+  }
 }
 
 }  // namespace dao

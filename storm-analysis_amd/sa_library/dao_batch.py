@@ -18,6 +18,7 @@ from .dao_fit_c import MultiFitterException, formatPeaksGaussianPSF
 
 SHAPES = {"2dfixed": 0, "2d": 1, "3d": 2, "Z": 3}
 ERROR_MODELS = {"MLE": 0, "ALS": 1, "LS": 2, "DWLS": 3, "FWLS": 4}
+PRECISIONS = {"fp64": 0, "mixed": 1}
 COUNTER_NAMES = ("n_dposv", "n_iterations", "n_lost", "n_margin", "n_neg_fi", "n_neg_height",
                  "n_non_converged", "n_non_decr")
 PEAK_COLUMNS = ("x", "y", "z", "height", "background", "xsigma", "ysigma", "error", "iterations")
@@ -34,7 +35,8 @@ class daoBatchModel(ctypes.Structure):
                 ("wx", ctypes.c_double * 5),
                 ("wy", ctypes.c_double * 5),
                 ("z_min", ctypes.c_double),
-                ("z_max", ctypes.c_double)]
+                ("z_max", ctypes.c_double),
+                ("precision", ctypes.c_int)]
 
 
 _lib = None
@@ -90,10 +92,13 @@ class BatchFitter(object):
     frame_shapes: list of (height, width); peak_caps: max peaks per frame.
     model: "2dfixed", "2d", "3d" or "Z"; error_model: "MLE", "ALS", "LS",
     "DWLS", "FWLS".  sigma_range, wx_params, wy_params, min_z, max_z as for
-    the MultiFitter classes.
+    the MultiFitter classes.  precision: "fp64" (the reference's arithmetic)
+    or "mixed" (FP32 per-pixel model, deviance terms, Jacobian and Hessian;
+    FP64 accumulators, solve and accept/reject; MLE only).
     """
     def __init__(self, frame_shapes, peak_caps, model="Z", error_model="MLE", roi_size=18, sigma_range=None,
-                 wx_params=None, wy_params=None, min_z=None, max_z=None, margin=2, tolerance=1.0e-6):
+                 wx_params=None, wy_params=None, min_z=None, max_z=None, margin=2, tolerance=1.0e-6,
+                 precision="fp64"):
         self.clib = loadBatchC()
         self.shapes = [tuple(s) for s in frame_shapes]
         self.n_frames = len(self.shapes)
@@ -103,6 +108,7 @@ class BatchFitter(object):
         m.roi_size = roi_size
         m.margin = margin
         m.tolerance = tolerance
+        m.precision = PRECISIONS[precision]
         if sigma_range is not None:
             m.width_max = 1.0 / (2.0 * sigma_range[0] * sigma_range[0])
             m.width_min = 1.0 / (2.0 * sigma_range[1] * sigma_range[1])
