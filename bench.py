@@ -73,6 +73,10 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
+    p.add_argument("--serve-seconds", type=float, default=3.0,
+                   help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
+    p.add_argument("--serve-loads", default="0.25,0.5,1.0",
+                   help="offered loads of the serving leg, as fractions of 16 cameras at U(50, 400) frames/s")
     p.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                    help="batched precisions to time (value = mixed when timed)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -244,6 +248,38 @@ def result_rows(outs):
     return rows, st
 
 
+def serve_leg(args, precision, log):
+    """
+    Continuous-batching server (8 workers x 32 slots of 1024^2 / 4000 peaks)
+    under 16 simulated cameras with Poisson arrivals, sensors of 256^2,
+    512^2 or 1024^2 and ~1000 emitters per 512^2, Z model; p50/p99 latency
+    and fits/s per offered load.
+    """
+    from storm_analysis_amd.sa_library import dao_serve
+    from storm_analysis_amd.diagnostics import serve_sim
+    pools = serve_sim.make_pools((256, 512, 1024))
+    srv_kw = dict(workers=8, slots=32, max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI,
+                  precision=precision, **zkw())
+    out = {"server": {k: v for k, v in srv_kw.items() if k in ("workers", "slots", "max_shape", "max_peaks",
+                                                               "precision")},
+           "cameras": 16, "rates_fps": [50, 400], "sizes": [256, 512, 1024], "runs": []}
+    # warm the kernels once
+    srv = dao_serve.FitServer(**srv_kw)
+    serve_sim.run(srv, duration_s=0.2, load=0.1, pools=pools)
+    srv.close()
+    for load in [float(x) for x in args.serve_loads.split(",") if x]:
+        srv = dao_serve.FitServer(**srv_kw)
+        try:
+            summ = serve_sim.run(srv, duration_s=args.serve_seconds, load=load, pools=pools)
+        finally:
+            srv.close()
+        log("serving load %.2f: %d frames, p50 %.1f ms, p99 %.1f ms, %.0f fits/s (offered %.0f emitters/s)" %
+            (load, summ["requests"], summ["p50_ms"], summ["p99_ms"], summ["fits_per_s"],
+             summ["offered_emitters_per_s"]))
+        out["runs"].append(summ)
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -317,6 +353,10 @@ def main():
             n = int(numpy.sum(per_frame[:nfr]))
             accuracy["%s_vs_oracle" % p] = agreement((out_p[:n], st_p[:n]), (orow, ost), D)
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serve_leg(args, head, log)
+
     hm = meas[head]
     line = {
         "metric": "converged emitter fits/sec + p99 frame latency; batched vs plain, 1 MI355X",
@@ -345,6 +385,7 @@ def main():
         "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
         "accuracy": accuracy,
         "mixed_xy_bar_px": TOL_MIXED_XY_PX,
+        "serving": serving,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }

@@ -48,6 +48,29 @@ typedef struct daoBatchModel {
                        (MLE only; other error models run in FP64) */
 } daoBatchModel;
 
+/* Serving.  The frame sizes given at creation are the slots' capacities; a
+ * slot can then hold any frame of at most that many pixels (and scheduling
+ * cells), one at a time. */
+
+/* Put a frame of width x height into slot `frame`: image, background, optional
+ * calibration (rqe and scmos_cal together, else rqe = 1 and scmos = 0) and
+ * n_rows start peaks in the daoNewPeaks layout.  Replaces what the slot held.
+ * Returns 0 (and changes nothing) if the frame or its peaks exceed the slot. */
+int daoBatchLoadFrame(daoBatch *b, int frame, int width, int height, const double *image,
+                      const double *background, const double *rqe, const double *scmos_cal, const double *rows,
+                      int n_rows, const char *p_type, int on_device);
+/* Empty slot `frame`; daoBatchFit skips empty slots. */
+void daoBatchClearFrame(daoBatch *b, int frame);
+/* daoBatchLoadFrame for n slots at once (uniform calibration: rqe = 1,
+ * scmos = 0), with one synchronisation: frames[k] receives the
+ * widths[k] x heights[k] image and background and n_rows[k] start peaks.
+ * Returns 0 (and changes nothing) if any frame does not fit its slot. */
+int daoBatchLoadFrames(daoBatch *b, int n, const int *frames, const int *widths, const int *heights,
+                       const double *const *images, const double *const *backgrounds, const double *const *rows,
+                       const int *n_rows, const char *p_type, int on_device);
+/* Empty n slots at once. */
+void daoBatchClearFrames(daoBatch *b, int n, const int *frames);
+
 /* Create a batch of n_frames frames; peak_caps[f] bounds the peaks of frame f.
  * Returns NULL without a GPU or with unsupported sizes. */
 daoBatch *daoBatchCreate(const daoBatchModel *model, int n_frames, const int *widths, const int *heights,

@@ -111,8 +111,9 @@ void daoBatchNewImages(daoBatch* b, const double* images, int on_device) {
   Engine& e = *b->eng;
   const double* d = images;
   if (!on_device) {
-    double* t = tmp(b, sizeof(double) * e.npix);
-    DAO_CHECK(hipMemcpyAsync(t, images, sizeof(double) * e.npix, hipMemcpyHostToDevice, e.st));
+    const long long n = e.frame_pixels_total();
+    double* t = tmp(b, sizeof(double) * n);
+    DAO_CHECK(hipMemcpyAsync(t, images, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
     d = t;
   }
   e.new_images_all_dev(d);
@@ -123,8 +124,9 @@ void daoBatchNewBackgrounds(daoBatch* b, const double* bgs, int on_device) {
   Engine& e = *b->eng;
   const double* d = bgs;
   if (!on_device) {
-    double* t = tmp(b, sizeof(double) * e.npix);
-    DAO_CHECK(hipMemcpyAsync(t, bgs, sizeof(double) * e.npix, hipMemcpyHostToDevice, e.st));
+    const long long n = e.frame_pixels_total();
+    double* t = tmp(b, sizeof(double) * n);
+    DAO_CHECK(hipMemcpyAsync(t, bgs, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
     d = t;
   }
   e.new_backgrounds_all_dev(d);
@@ -137,19 +139,135 @@ void daoBatchAnscombe(daoBatch* b) {
   e.sync();
 }
 
+static int peak_format(const char* p_type) {
+  if (!std::strcmp(p_type, "finder")) return dao::PF_FINDER;
+  if (!std::strcmp(p_type, "testing")) return dao::PF_TESTING;
+  if (!std::strcmp(p_type, "text") || !std::strcmp(p_type, "hdf5")) return dao::PF_TEXT;
+  std::printf("Unrecognized peak type '%s'!\n", p_type);
+  return -1;
+}
+
+int daoBatchLoadFrame(daoBatch* b, int f, int width, int height, const double* image, const double* background,
+                      const double* rqe, const double* scmos_cal, const double* rows, int n_rows, const char* p_type,
+                      int on_device) {
+  Engine& e = *b->eng;
+  const int fmt = peak_format(p_type);
+  if (f < 0 || f >= e.F || fmt < 0 || n_rows < 0 || n_rows > e.fr[f].cap) return 0;
+  if (!e.set_frame_shape(f, width, height)) return 0;
+  const long long n = (long long)width * height;
+  const int wid = fmt == dao::PF_TEXT ? 7 : 4;
+  // staging: image, background, calibration, rows
+  const double *img = image, *bg = background, *rq = rqe, *sc = scmos_cal, *rw = rows;
+  if (!on_device) {
+    double* t = tmp(b, sizeof(double) * (4 * n + (long long)n_rows * wid));
+    DAO_CHECK(hipMemcpyAsync(t, image, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+    DAO_CHECK(hipMemcpyAsync(t + n, background, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+    img = t;
+    bg = t + n;
+    if (rqe) {
+      DAO_CHECK(hipMemcpyAsync(t + 2 * n, rqe, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+      rq = t + 2 * n;
+    }
+    if (scmos_cal) {
+      DAO_CHECK(hipMemcpyAsync(t + 3 * n, scmos_cal, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+      sc = t + 3 * n;
+    }
+    if (n_rows > 0) {
+      DAO_CHECK(hipMemcpyAsync(t + 4 * n, rows, sizeof(double) * n_rows * wid, hipMemcpyHostToDevice, e.st));
+      rw = t + 4 * n;
+    }
+  }
+  if (rq && sc) {
+    dao::launch_set_calibration(e.px, e.fr[f].pix0, n, rq, sc, e.st);
+  } else {
+    e.set_calibration_const(f, 1.0, 0.0);
+    if (rq || sc) std::printf("daoBatchLoadFrame: rqe and scmos_cal go together; using rqe = 1, scmos = 0\n");
+  }
+  e.new_image_dev(f, img);
+  e.new_background_dev(f, bg);
+  if (e.m.errm == dao::EM_ALS) dao::launch_anscombe(e.px, e.fr[f].pix0, n, e.st);
+  if (n_rows > 0) e.new_peaks_multi({f}, {rw}, {n_rows}, fmt);
+  e.sync();
+  pull_counters(b);
+  return 1;
+}
+
+int daoBatchLoadFrames(daoBatch* b, int n, const int* frames, const int* widths, const int* heights,
+                       const double* const* images, const double* const* backgrounds, const double* const* rows,
+                       const int* n_rows, const char* p_type, int on_device) {
+  Engine& e = *b->eng;
+  const int fmt = peak_format(p_type);
+  if (fmt < 0 || n < 0) return 0;
+  const int wid = fmt == dao::PF_TEXT ? 7 : 4;
+  for (int k = 0; k < n; k++) {
+    const int f = frames[k];
+    if (f < 0 || f >= e.F || n_rows[k] < 0 || n_rows[k] > e.fr[f].cap) return 0;
+    const int cell = e.sc.cell;
+    const long long px = (long long)widths[k] * heights[k];
+    const int cells = ((widths[k] + cell - 1) / cell) * ((heights[k] + cell - 1) / cell);
+    if (widths[k] < 1 || heights[k] < 1 || px > e.pix_cap[f] || cells > e.cell_cap[f]) return 0;
+  }
+  // host arrays go through one staging buffer
+  std::vector<const double*> img(images, images + n), bg(backgrounds, backgrounds + n), rw(rows, rows + n);
+  if (!on_device) {
+    long long total = 0;
+    for (int k = 0; k < n; k++) total += 2LL * widths[k] * heights[k] + (long long)n_rows[k] * wid;
+    double* t = tmp(b, sizeof(double) * (total + 1));
+    for (int k = 0; k < n; k++) {
+      const long long px = (long long)widths[k] * heights[k];
+      DAO_CHECK(hipMemcpyAsync(t, images[k], sizeof(double) * px, hipMemcpyHostToDevice, e.st));
+      img[k] = t;
+      t += px;
+      DAO_CHECK(hipMemcpyAsync(t, backgrounds[k], sizeof(double) * px, hipMemcpyHostToDevice, e.st));
+      bg[k] = t;
+      t += px;
+      if (n_rows[k] > 0) {
+        DAO_CHECK(hipMemcpyAsync(t, rows[k], sizeof(double) * n_rows[k] * wid, hipMemcpyHostToDevice, e.st));
+        rw[k] = t;
+        t += (long long)n_rows[k] * wid;
+      }
+    }
+  }
+  for (int k = 0; k < n; k++) e.set_frame_shape(frames[k], widths[k], heights[k], false);
+  e.upload_frames();
+  std::vector<int> fs, cnt;
+  std::vector<const double*> rp;
+  for (int k = 0; k < n; k++) {
+    const int f = frames[k];
+    const long long px = (long long)widths[k] * heights[k];
+    e.set_calibration_const(f, 1.0, 0.0);
+    dao::launch_new_image(e.px, e.fr[f].pix0, px, img[k], e.st);
+    dao::launch_new_background(e.px, e.fr[f].pix0, px, bg[k], e.st);
+    if (e.m.errm == dao::EM_ALS) dao::launch_anscombe(e.px, e.fr[f].pix0, px, e.st);
+    if (n_rows[k] > 0) {
+      fs.push_back(f);
+      cnt.push_back(n_rows[k]);
+      rp.push_back(rw[k]);
+    }
+  }
+  if (!fs.empty()) e.new_peaks_multi(fs, rp, cnt, fmt);
+  e.sync();
+  pull_counters(b);
+  return 1;
+}
+
+void daoBatchClearFrames(daoBatch* b, int n, const int* frames) {
+  Engine& e = *b->eng;
+  for (int k = 0; k < n; k++)
+    if (frames[k] >= 0 && frames[k] < e.F) e.fr[frames[k]].nfit = 0;
+  e.upload_frames();
+}
+
+void daoBatchClearFrame(daoBatch* b, int f) {
+  Engine& e = *b->eng;
+  if (f < 0 || f >= e.F) return;
+  e.clear_frame(f);
+}
+
 void daoBatchNewPeaks(daoBatch* b, const double* rows, const int* counts, const char* p_type, int on_device) {
   Engine& e = *b->eng;
-  int fmt;
-  if (!std::strcmp(p_type, "finder")) {
-    fmt = dao::PF_FINDER;
-  } else if (!std::strcmp(p_type, "testing")) {
-    fmt = dao::PF_TESTING;
-  } else if (!std::strcmp(p_type, "text") || !std::strcmp(p_type, "hdf5")) {
-    fmt = dao::PF_TEXT;
-  } else {
-    std::printf("Unrecognized peak type '%s'!\n", p_type);
-    return;
-  }
+  const int fmt = peak_format(p_type);
+  if (fmt < 0) return;
   int width = fmt == dao::PF_TEXT ? 7 : 4;
   long long total = 0;
   for (int f = 0; f < e.F; f++) total += counts[f];

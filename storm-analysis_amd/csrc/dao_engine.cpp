@@ -85,6 +85,8 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
     pix += (long long)q.W * q.H;
     slots += q.cap;
     cells += q.ncx * q.ncy;
+    pix_cap.push_back((long long)q.W * q.H);
+    cell_cap.push_back(q.ncx * q.ncy);
   }
   npix = pix;
   ncells = cells;
@@ -698,12 +700,69 @@ void Engine::pack_all(double* out, int32_t* status) {
   sync();
 }
 
+bool Engine::packed() const {
+  for (int f = 0; f < F; f++)
+    if ((long long)fr[f].W * fr[f].H != pix_cap[f]) return false;
+  return true;
+}
+
+// Frames' data concatenated in frame order at their current sizes.
 void Engine::new_images_all_dev(const double* img) {
-  launch_new_image(px, 0, npix, img, st);
+  if (packed()) {
+    launch_new_image(px, 0, npix, img, st);
+  } else {
+    long long at = 0;
+    for (int f = 0; f < F; f++) {
+      long long n = (long long)fr[f].W * fr[f].H;
+      launch_new_image(px, fr[f].pix0, n, img + at, st);
+      at += n;
+    }
+  }
   for (int f = 0; f < F; f++) fr[f].nfit = 0;
   upload_frames();
 }
 
-void Engine::new_backgrounds_all_dev(const double* bg) { launch_new_background(px, 0, npix, bg, st); }
+void Engine::new_backgrounds_all_dev(const double* bg) {
+  if (packed()) {
+    launch_new_background(px, 0, npix, bg, st);
+    return;
+  }
+  long long at = 0;
+  for (int f = 0; f < F; f++) {
+    long long n = (long long)fr[f].W * fr[f].H;
+    launch_new_background(px, fr[f].pix0, n, bg + at, st);
+    at += n;
+  }
+}
+
+long long Engine::frame_pixels_total() const {
+  long long n = 0;
+  for (int f = 0; f < F; f++) n += (long long)fr[f].W * fr[f].H;
+  return n;
+}
+
+// A new size for slot f within the pixel and cell capacity it was created
+// with; the slot is emptied.
+bool Engine::set_frame_shape(int f, int W, int H, bool upload) {
+  const int cell = sc.cell;
+  const int ncx = (W + cell - 1) / cell, ncy = (H + cell - 1) / cell;
+  if (W < 1 || H < 1 || (long long)W * H > pix_cap[f] || ncx * ncy > cell_cap[f]) return false;
+  fr[f].W = W;
+  fr[f].H = H;
+  fr[f].ncx = ncx;
+  fr[f].ncy = ncy;
+  fr[f].nfit = 0;
+  if (upload) upload_frames();
+  return true;
+}
+
+void Engine::clear_frame(int f) {
+  fr[f].nfit = 0;
+  upload_frames();
+}
+
+void Engine::set_calibration_const(int f, double rqe, double scmos) {
+  launch_set_calibration_const(px, fr[f].pix0, (long long)fr[f].W * fr[f].H, rqe, scmos, st);
+}
 
 }  // namespace dao

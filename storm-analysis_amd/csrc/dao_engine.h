@@ -32,6 +32,8 @@ void launch_new_image(const Pix& px, long long pix0, long long n, const double* 
 void launch_new_background(const Pix& px, long long pix0, long long n, const double* bg, hipStream_t st);
 void launch_set_calibration(const Pix& px, long long pix0, long long n, const double* rqe, const double* scm,
                             hipStream_t st);
+void launch_set_calibration_const(const Pix& px, long long pix0, long long n, double rqe, double scm,
+                                  hipStream_t st);
 void launch_anscombe(const Pix& px, long long pix0, long long n, hipStream_t st);
 void launch_fit_image(const Pix& px, long long pix0, long long n, double* out, int residual, hipStream_t st);
 void launch_peak_sums(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames, const int* roi_tab,
@@ -140,9 +142,18 @@ class Engine {
   // all frames' live peaks as rows of 9 doubles + status (host buffers)
   void pack_all(double* out_host, int32_t* status_host);
   long long total_nfit() const;
-  // every frame at once from concatenated device arrays
+  // every frame at once from device arrays concatenated at the frames'
+  // current sizes
   void new_images_all_dev(const double* img_dev);
   void new_backgrounds_all_dev(const double* bg_dev);
+  long long frame_pixels_total() const;
+  // serving: re-size slot f within its capacity (false if it does not fit),
+  // empty it, uniform calibration
+  bool set_frame_shape(int f, int W, int H, bool upload = true);
+  void clear_frame(int f);
+  void set_calibration_const(int f, double rqe, double scmos);
+  std::vector<long long> pix_cap;  // pixel capacity of each slot
+  std::vector<int> cell_cap;       // scheduling-cell capacity of each slot
   int count_status(int f, int status);
   void sync();
 
@@ -154,6 +165,7 @@ class Engine {
   double* stage(size_t bytes);
   int* istage(size_t bytes);
   int select(int mode);
+  bool packed() const;  // every frame at its creation size
   void run_ops(int kind, const double* rows, int fmt, int first_slot);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* d_sweeps = nullptr;

@@ -254,6 +254,14 @@ __global__ void k_set_calibration(Pix px, long long pix0, long long n, const dou
   px.ro[pix0 + i].scm = scm[i];
 }
 
+// Uniform calibration of pixels [pix0, pix0 + n).
+__global__ void k_set_calibration_const(Pix px, long long pix0, long long n, double rqe, double scm) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  px.ro[pix0 + i].rqe = rqe;
+  px.ro[pix0 + i].scm = scm;
+}
+
 // mFitAnscombeTransformImage (multi_fit.c:90-97).
 __global__ void k_anscombe(Pix px, long long pix0, long long n) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -613,6 +621,10 @@ void launch_new_background(const Pix& px, long long pix0, long long n, const dou
 void launch_set_calibration(const Pix& px, long long pix0, long long n, const double* rqe, const double* scm,
                             hipStream_t st) {
   k_set_calibration<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, rqe, scm);
+}
+void launch_set_calibration_const(const Pix& px, long long pix0, long long n, double rqe, double scm,
+                                  hipStream_t st) {
+  k_set_calibration_const<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, rqe, scm);
 }
 void launch_anscombe(const Pix& px, long long pix0, long long n, hipStream_t st) {
   k_anscombe<<<flat_grid(n), 256, 0, st>>>(px, pix0, n);

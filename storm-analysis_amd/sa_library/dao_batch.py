@@ -66,6 +66,14 @@ def loadBatchC():
         lib.daoBatchCounters.argtypes = [vp, ctypes.c_int, i32p]
         lib.daoBatchGetFitImage.argtypes = [vp, ctypes.c_int, f64p]
         lib.daoBatchRemoveErrorPeaks.argtypes = [vp]
+        lib.daoBatchLoadFrame.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
+                                          ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        lib.daoBatchLoadFrame.restype = ctypes.c_int
+        lib.daoBatchClearFrame.argtypes = [vp, ctypes.c_int]
+        lib.daoBatchLoadFrames.argtypes = [vp, ctypes.c_int, i32p, i32p, i32p, vp, vp, vp, i32p, ctypes.c_char_p,
+                                           ctypes.c_int]
+        lib.daoBatchLoadFrames.restype = ctypes.c_int
+        lib.daoBatchClearFrames.argtypes = [vp, ctypes.c_int, i32p]
         lib.daoBatchSetTiming.argtypes = [vp, ctypes.c_int]
         lib.daoBatchStats.argtypes = [vp, f64p]
         _lib = lib
@@ -79,8 +87,10 @@ def _ptr(a, dtype):
             import torch
             if not a.is_contiguous() or a.dtype != torch.float64:
                 raise MultiFitterException("device arrays must be contiguous float64")
-            # the library works on its own stream: finish torch's writes first
-            torch.cuda.synchronize(a.device)
+            # the library works on its own streams: finish the writes queued on
+            # torch's current stream (not the whole device, which would also
+            # wait for other fitters' work)
+            torch.cuda.current_stream(a.device).synchronize()
             return ctypes.c_void_p(a.data_ptr()), 1, a
         a = a.numpy()
     a = numpy.ascontiguousarray(a, dtype=dtype)
@@ -93,8 +103,9 @@ class BatchFitter(object):
     model: "2dfixed", "2d", "3d" or "Z"; error_model: "MLE", "ALS", "LS",
     "DWLS", "FWLS".  sigma_range, wx_params, wy_params, min_z, max_z as for
     the MultiFitter classes.  precision: "fp64" (the reference's arithmetic)
-    or "mixed" (FP32 per-pixel model, deviance terms, Jacobian and Hessian;
-    FP64 accumulators, solve and accept/reject; MLE only).
+    or "mixed" (FP32 deviance terms, Jacobian terms and Hessian; FP64 peak
+    shape, fit-image accumulators, deviance and Jacobian sums, solve and
+    accept/reject; MLE only, other error models run in FP64).
     """
     def __init__(self, frame_shapes, peak_caps, model="Z", error_model="MLE", roi_size=18, sigma_range=None,
                  wx_params=None, wy_params=None, min_z=None, max_z=None, margin=2, tolerance=1.0e-6,
@@ -194,6 +205,87 @@ class BatchFitter(object):
             raise MultiFitterException("more peaks than the frame's capacity")
         p, dev, keep = _ptr(rows, numpy.float64)
         self.clib.daoBatchNewPeaks(self.handle, p, counts, ctypes.c_char_p(peaks_type.encode()), dev)
+
+    def loadFrame(self, frame, image, background, rows, peaks_type="hdf5", rqe=None, scmos_cal=None):
+        """
+        Serving: put one frame (any size up to the slot's creation size) with
+        its background, start peaks (rows in the daoNewPeaks layout) and
+        optional calibration into slot `frame`.  Arrays are numpy (host) or
+        torch CUDA tensors (all of one kind).
+        """
+        h, w = image.shape[0], image.shape[1]
+        width = 7 if peaks_type in ("text", "hdf5") else 4
+        n = rows.shape[0] if rows is not None else 0
+        if rows is not None and rows.shape[1] != width:
+            raise MultiFitterException("peak rows must be n x %d for '%s'" % (width, peaks_type))
+        keep = []
+        ptrs = []
+        on_device = None
+        for a in (image, background, rqe, scmos_cal, rows if n else None):
+            if a is None:
+                ptrs.append(None)
+                continue
+            p, dev, k = _ptr(a if hasattr(a, "data_ptr") else numpy.ascontiguousarray(a, dtype=numpy.float64),
+                             numpy.float64)
+            if on_device is None:
+                on_device = dev
+            elif dev != on_device:
+                raise MultiFitterException("loadFrame arrays must all be host or all device arrays")
+            keep.append(k)
+            ptrs.append(p)
+        ok = self.clib.daoBatchLoadFrame(self.handle, frame, w, h, ptrs[0], ptrs[1], ptrs[2], ptrs[3], ptrs[4], n,
+                                         ctypes.c_char_p(peaks_type.encode()), 1 if on_device else 0)
+        if not ok:
+            raise MultiFitterException("frame %dx%d with %d peaks does not fit slot %d" % (h, w, n, frame))
+        self.shapes[frame] = (h, w)
+
+    def clearFrame(self, frame):
+        """Serving: empty slot `frame` (doFit skips it)."""
+        self.clib.daoBatchClearFrame(self.handle, frame)
+
+    def loadFrames(self, slots, images, backgrounds, rows, peaks_type="hdf5"):
+        """
+        Serving: loadFrame for several slots in one call (rqe = 1, scmos = 0).
+        images, backgrounds, rows: sequences (one entry per slot) of numpy
+        arrays or of torch CUDA tensors (all of one kind).
+        """
+        n = len(slots)
+        if n == 0:
+            return
+        width = 7 if peaks_type in ("text", "hdf5") else 4
+        keep = []
+        ptr = {"img": (ctypes.c_void_p * n)(), "bg": (ctypes.c_void_p * n)(), "rows": (ctypes.c_void_p * n)()}
+        on_device = None
+        nr = numpy.zeros(n, dtype=numpy.int32)
+        w = numpy.zeros(n, dtype=numpy.int32)
+        h = numpy.zeros(n, dtype=numpy.int32)
+        for k in range(n):
+            h[k], w[k] = images[k].shape[0], images[k].shape[1]
+            nr[k] = rows[k].shape[0] if rows[k] is not None else 0
+            if nr[k] and rows[k].shape[1] != width:
+                raise MultiFitterException("peak rows must be n x %d for '%s'" % (width, peaks_type))
+            for key, a in (("img", images[k]), ("bg", backgrounds[k]), ("rows", rows[k] if nr[k] else None)):
+                if a is None:
+                    continue
+                p, dev, kp = _ptr(a, numpy.float64)
+                if on_device is None:
+                    on_device = dev
+                elif dev != on_device:
+                    raise MultiFitterException("loadFrames arrays must all be host or all device arrays")
+                keep.append(kp)
+                ptr[key][k] = p.value
+        ok = self.clib.daoBatchLoadFrames(self.handle, n, numpy.asarray(slots, dtype=numpy.int32), w, h,
+                                          ptr["img"], ptr["bg"], ptr["rows"], nr,
+                                          ctypes.c_char_p(peaks_type.encode()), 1 if on_device else 0)
+        if not ok:
+            raise MultiFitterException("a frame or its peaks do not fit its slot")
+        for k, f in enumerate(slots):
+            self.shapes[f] = (int(h[k]), int(w[k]))
+
+    def clearFrames(self, slots):
+        """Serving: empty several slots in one call."""
+        if len(slots):
+            self.clib.daoBatchClearFrames(self.handle, len(slots), numpy.asarray(slots, dtype=numpy.int32))
 
     def doFit(self, max_iterations=2000):
         return self.clib.daoBatchFit(self.handle, max_iterations)

@@ -1,0 +1,198 @@
+"""
+Serving: frames from many acquisitions fitted as they arrive.
+
+The reference fits one frame per MultiFitter, frame after frame
+(dao_fit_c.py:257-549, driven by fitting.PeakFitter, fitting.py:701-824).
+FitServer keeps a fixed pool of frame slots resident on the GPU -- `workers`
+batch fitters of `slots` slots each, every slot sized for the largest frame
+it accepts -- and runs continuous batching over it:
+
+- admission: the slot pool is sized from an HBM budget; a request waits in
+  the queue until a worker has free slots, and frames larger than a slot are
+  refused at submit;
+- each worker, on its own host thread and HIP stream, takes whatever
+  requests are queued (up to its slot count), loads them into its slots,
+  fits them in one persistent launch (MultiFitter.doFit for every slot,
+  dao_fit_c.py:332-356) and hands back the results; while one worker fits,
+  the others load and fit the next requests, so work from different
+  requests overlaps on the device;
+- latency = result ready - arrival, per request.
+
+Frames share no state in the fitter, so a frame's result does not depend on
+which other frames share its launch (to the parity bar; bit-identical unless
+a reserved box overflowed and a rollback ran).
+"""
+import queue
+import threading
+import time
+
+import numpy
+
+from . import dao_batch
+from .dao_fit_c import MultiFitterException
+
+# device bytes per slot pixel (two 32-byte pixel records + the Anscombe
+# image) and per slot peak (two 96-byte peak records + scheduling scratch)
+BYTES_PER_PIXEL = 72
+BYTES_PER_PEAK = 840
+
+
+def slot_bytes(max_shape, max_peaks):
+    """Device bytes one slot of max_shape pixels and max_peaks peaks holds."""
+    return max_shape[0] * max_shape[1] * BYTES_PER_PIXEL + max_peaks * BYTES_PER_PEAK
+
+
+def pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb=None):
+    """(workers, slots) after admission by the HBM budget (fewer slots first)."""
+    if hbm_budget_gb is None:
+        return workers, slots
+    fit = int(hbm_budget_gb * 1e9 // slot_bytes(max_shape, max_peaks))
+    if fit < 1:
+        raise MultiFitterException("HBM budget holds no slot of %s" % (tuple(max_shape),))
+    while workers * slots > fit and slots > 1:
+        slots -= 1
+    return max(1, min(workers, fit // slots)), slots
+
+
+class FitRequest(object):
+    """One frame to fit; `result` (peaks dict, as BatchFitter.getPeaks) once `done` is set."""
+    def __init__(self, image, background, rows, peaks_type, camera, t_arrival):
+        self.image = image
+        self.background = background
+        self.rows = rows
+        self.peaks_type = peaks_type
+        self.camera = camera
+        self.t_arrival = t_arrival
+        self.t_start = None
+        self.t_done = None
+        self.worker = None
+        self.batch_size = None
+        self.result = None
+        self.error = None
+        self.done = threading.Event()
+
+    def latency(self):
+        return self.t_done - self.t_arrival
+
+    def wait(self, timeout=None):
+        if not self.done.wait(timeout):
+            raise TimeoutError("fit request not done")
+        if self.error is not None:
+            raise self.error
+        return self.result
+
+
+class FitServer(object):
+    """
+    workers x slots frame slots of max_shape (height, width) and max_peaks
+    peaks each.  hbm_budget_gb, when given, caps workers x slots to what the
+    budget holds.  The remaining keywords are BatchFitter's (model,
+    precision, roi_size, wx_params, ...).
+    """
+    def __init__(self, workers=4, slots=32, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None,
+                 max_iterations=2000, **fitter_kw):
+        self.max_shape = tuple(max_shape)
+        self.max_peaks = max_peaks
+        self.max_iterations = max_iterations
+        workers, slots = pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb)
+        self.workers = workers
+        self.slots = slots
+        self.slot_bytes = slot_bytes(max_shape, max_peaks)
+        self.fitters = [dao_batch.BatchFitter([self.max_shape] * slots, [max_peaks] * slots, **fitter_kw)
+                        for _ in range(workers)]
+        self.queue = queue.Queue()
+        self.completed = []
+        self._lock = threading.Lock()
+        self.launches = 0
+        self.threads = [threading.Thread(target=self._work, args=(w,), daemon=True) for w in range(workers)]
+        for t in self.threads:
+            t.start()
+
+    def submit(self, image, background, rows, peaks_type="hdf5", camera=0, t_arrival=None):
+        """Queue one frame; returns its FitRequest."""
+        h, w = image.shape[0], image.shape[1]
+        if h * w > self.max_shape[0] * self.max_shape[1] or h > self.max_shape[0] or w > self.max_shape[1]:
+            raise MultiFitterException("frame %dx%d exceeds the slot size %s" % (h, w, self.max_shape))
+        if rows is not None and rows.shape[0] > self.max_peaks:
+            raise MultiFitterException("%d peaks exceed the slot's %d" % (rows.shape[0], self.max_peaks))
+        r = FitRequest(image, background, rows, peaks_type, camera,
+                       time.perf_counter() if t_arrival is None else t_arrival)
+        self.queue.put(r)
+        return r
+
+    def _work(self, w):
+        bf = self.fitters[w]
+        used = 0
+        while True:
+            first = self.queue.get()
+            if first is None:
+                return
+            batch = [first]
+            while len(batch) < self.slots:
+                try:
+                    r = self.queue.get_nowait()
+                except queue.Empty:
+                    break
+                if r is None:
+                    self.queue.put(None)
+                    break
+                batch.append(r)
+            t0 = time.perf_counter()
+            try:
+                for r in batch:
+                    r.t_start = t0
+                # one load call per peak format (normally one)
+                for ptype in sorted(set(r.peaks_type for r in batch)):
+                    ks = [k for k, r in enumerate(batch) if r.peaks_type == ptype]
+                    bf.loadFrames(ks, [batch[k].image for k in ks], [batch[k].background for k in ks],
+                                  [batch[k].rows for k in ks], ptype)
+                bf.clearFrames(list(range(len(batch), used)))
+                used = len(batch)
+                bf.doFit(self.max_iterations)
+                rows, status, counts = bf.getAllPeaks()
+                at = numpy.concatenate([[0], numpy.cumsum(counts)])
+                for k, r in enumerate(batch):
+                    a, b = int(at[k]), int(at[k + 1])
+                    res = {c: rows[a:b, i].copy() for i, c in enumerate(dao_batch.PEAK_COLUMNS)}
+                    res["iterations"] = res["iterations"].astype(numpy.int32)
+                    res["status"] = status[a:b].copy()
+                    r.result = res
+            except Exception as e:  # reported to every request of the batch
+                for r in batch:
+                    r.error = e
+            t1 = time.perf_counter()
+            with self._lock:
+                self.launches += 1
+                for r in batch:
+                    r.t_done = t1
+                    r.worker = w
+                    r.batch_size = len(batch)
+                    self.completed.append(r)
+            for r in batch:
+                r.done.set()
+
+    def close(self):
+        for _ in self.threads:
+            self.queue.put(None)
+        for t in self.threads:
+            t.join()
+        for bf in self.fitters:
+            bf.cleanup()
+        self.fitters = []
+
+    def summary(self, t_begin=None):
+        """Latency percentiles (ms), throughput and batching figures of the completed requests."""
+        with self._lock:
+            done = list(self.completed)
+        if not done:
+            return {"requests": 0}
+        lat = numpy.array([r.latency() for r in done]) * 1e3
+        t0 = min(r.t_arrival for r in done) if t_begin is None else t_begin
+        t1 = max(r.t_done for r in done)
+        conv = sum(int((r.result["status"] == 1).sum()) for r in done if r.result is not None)
+        return {"requests": len(done), "p50_ms": float(numpy.percentile(lat, 50)),
+                "p99_ms": float(numpy.percentile(lat, 99)), "max_ms": float(lat.max()),
+                "converged": conv, "fits_per_s": conv / (t1 - t0), "frames_per_s": len(done) / (t1 - t0),
+                "launches": self.launches, "mean_batch": float(numpy.mean([r.batch_size for r in done])),
+                "workers": self.workers, "slots": self.slots, "slot_bytes": self.slot_bytes,
+                "errors": sum(1 for r in done if r.error is not None)}
