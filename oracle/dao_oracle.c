@@ -27,12 +27,13 @@
  *   significance, RQE    test_mfit_9,12
  *   LM state machine     test_mfit_5 (two-peak fit), noise-free convergence to
  *                        the truth for every model x error model, monotone
- *                        accepted errors, counter bookkeeping invariants,
- *                        and a second, independently written pure-Python
- *                        restatement on tiny frames (tests/golden/lm_pure.py).
- * The compiled reference is not used: its ROI-table fill
- * (multi_fit.c:1052-1056) is empty in the mounted tree and it needs an
- * external LAPACK, so it cannot serve as an oracle here.
+ *                        accepted errors of an isolated peak, counter
+ *                        bookkeeping invariants, and a second, independently
+ *                        written pure-Python restatement on tiny frames
+ *                        (tests/lm_pure.py, tests/test_oracle_lm.py).
+ * The compiled reference is not used: multi_fit.c needs an external LAPACK
+ * (dposv_, multi_fit.c:17-18,1851) that is not among the reference's sources,
+ * so it is not built here.
  *
  * Quirks of the reference that are kept on purpose:
  *   - the per-pixel t_fi / err_i cache with stale flags (multi_fit.c:111-195):
