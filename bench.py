@@ -77,6 +77,8 @@ def parse():
                    help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
     p.add_argument("--serve-loads", default="0.25,0.5,1.0",
                    help="offered loads of the serving leg, as fractions of 16 cameras at U(50, 400) frames/s")
+    p.add_argument("--no-uneven", action="store_true",
+                   help="skip the batched-uneven-frames leg (sCMOS 3D, 128^2..2048^2, autotuned)")
     p.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
                    help="batched precisions to time (value = mixed when timed)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -248,6 +250,58 @@ def result_rows(outs):
     return rows, st
 
 
+def uneven_leg(log, seed=20261209):
+    """
+    Batched uneven frames in one launch with sCMOS weighting, '3d' model (free
+    wx, wy), sigma U(1.2, 2.0), ROI 18: 8 frames of 128^2, 8 of 256^2, 4 of
+    512^2 at densities U(0.1, 3) emitters/um^2 (160 nm pixels) and one 2048^2
+    frame at 0.2 /um^2; per-pixel offset N(100, 5), gain N(2, 0.1), read
+    variance lognormal(ln 4, 0.8).  The autotuner picks waves per workgroup
+    and the reserved-box margin; FP64 and mixed precision are timed with the
+    pick.
+    """
+    import numpy as np
+    from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D, dao_tune
+    from storm_analysis_amd.diagnostics import synth
+    rng = np.random.default_rng(seed)
+    spec = [(128, d) for d in rng.uniform(0.1, 3.0, 8)] + [(256, d) for d in rng.uniform(0.1, 3.0, 8)] + \
+           [(512, d) for d in rng.uniform(0.1, 3.0, 4)] + [(2048, 0.2)]
+    t0 = time.perf_counter()
+    frames = [synth.frame_3d_scmos(rng, size=s, density=d) for s, d in spec]
+    log("uneven: %d frames, %d emitters generated in %.1f s" %
+        (len(frames), sum(len(f.peaks["x"]) for f in frames), time.perf_counter() - t0))
+    kw = {"roi_size": ROI, "sigma_range": [0.6, 10.0]}
+    bgs = [np.full(f.image.shape, 100.0) for f in frames]
+    out = {"frames": [[s, round(float(d), 3), int(len(f.peaks["x"]))] for (s, d), f in zip(spec, frames)],
+           "model": "3d", "calibration": "sCMOS"}
+    for prec in ("mixed", "fp64"):
+        bf = dao_batch.BatchFitter([f.image.shape for f in frames], [len(f.peaks["x"]) for f in frames],
+                                   model="3d", precision=prec, **kw)
+
+        def prepare():
+            for k, f in enumerate(frames):
+                bf.setCalibration(k, rqe=f.rqe, scmos_cal=f.scmos_cal)
+            bf.newImages([f.image for f in frames])
+            bf.newBackgrounds(bgs)
+            bf.newPeaks([f.peaks for f in frames], "hdf5")
+
+        if prec == "mixed":
+            best, trials = dao_tune.autotune(bf, prepare, waves=(2, 4, 8), margins=(1, 2, 3), log=log)
+            out["autotune"] = {"best": {"waves": best[0], "margin": best[1]}, "trials": trials}
+        else:
+            bf.setTuning(out["autotune"]["best"]["waves"], out["autotune"]["best"]["margin"])
+        prepare()
+        t0 = time.perf_counter()
+        bf.doFit()
+        rows, status, counts = bf.getAllPeaks()
+        dt = time.perf_counter() - t0
+        conv = int((status == D.CONVERGED).sum())
+        out[prec] = {"fit_and_readback_s": dt, "converged": conv, "fits_per_s": conv / dt}
+        log("uneven %s: %d converged in %.3f s (%.0f fits/s)" % (prec, conv, dt, conv / dt))
+        bf.cleanup()
+    return out
+
+
 def serve_leg(args, precision, log):
     """
     Continuous-batching server (8 workers x 32 slots of 1024^2 / 4000 peaks)
@@ -353,6 +407,10 @@ def main():
             n = int(numpy.sum(per_frame[:nfr]))
             accuracy["%s_vs_oracle" % p] = agreement((out_p[:n], st_p[:n]), (orow, ost), D)
 
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_leg(log)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serve_leg(args, head, log)
@@ -385,6 +443,7 @@ def main():
         "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
         "accuracy": accuracy,
         "mixed_xy_bar_px": TOL_MIXED_XY_PX,
+        "uneven_batch": uneven,
         "serving": serving,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,

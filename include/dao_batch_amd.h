@@ -110,6 +110,12 @@ void daoBatchRemoveErrorPeaks(daoBatch *b);
 /* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
  * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */
 void daoBatchSetTiming(daoBatch *b, int on);
+/* Tuning of the frame-per-workgroup fitter: waves per workgroup (0: the
+ * kernel's maximum, 8 for ROIs up to 512 pixels, else 4; fewer waves leave
+ * room for more frames per CU) and the reserved-box margin in pixels (>= 0;
+ * -1 keeps it).  Results are the same for every setting (bit-identical unless
+ * a reserved box overflows and the rollback's arithmetic undo runs). */
+void daoBatchSetTuning(daoBatch *b, int waves, int margin);
 void daoBatchStats(daoBatch *b, double *out6);
 
 #ifdef __cplusplus

@@ -258,6 +258,14 @@ void daoBatchClearFrames(daoBatch* b, int n, const int* frames) {
   e.upload_frames();
 }
 
+void daoBatchSetTuning(daoBatch* b, int waves, int margin) {
+  Engine& e = *b->eng;
+  dao::Model m = e.m;
+  m.waves = waves < 0 ? 0 : waves;
+  if (margin >= 0) m.margin = margin;
+  e.set_model(m);
+}
+
 void daoBatchClearFrame(daoBatch* b, int f) {
   Engine& e = *b->eng;
   if (f < 0 || f >= e.F) return;

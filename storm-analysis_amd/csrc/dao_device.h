@@ -45,6 +45,7 @@ struct Model {
   int jac;       // jac_size
   int margin;    // reserved-box margin around the ROI, pixels
   int prec;      // Precision
+  int waves;     // waves per workgroup of k_fit_frames (0: the kernel's maximum)
   double xoff;   // == yoff
   double zoff;
   double tol;

@@ -412,6 +412,7 @@ int Engine::fit_frames(int max_more) {
   }
   if (timing) DAO_CHECK(hipEventRecord(ev0, st));
   launch_fit_frames(a, (int)list.size(), st);
+  DAO_CHECK(hipGetLastError());
   if (timing) DAO_CHECK(hipEventRecord(ev1, st));
   DAO_CHECK(hipMemcpyAsync(h, d_sweeps, sizeof(int) * F, hipMemcpyDeviceToHost, st));
   DAO_CHECK(hipMemcpyAsync(h + F, d_err, sizeof(int), hipMemcpyDeviceToHost, st));

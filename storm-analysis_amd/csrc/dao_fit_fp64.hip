@@ -7,7 +7,7 @@ namespace dao {
 void launch_fit_frames_fp64(const FitArgs& a, int nframes, hipStream_t st) {
   DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, ({
                    constexpr int NW = PPL <= 8 ? 8 : 4;
-                   k_fit_frames<PPL, SHP, NW, false><<<nframes, NW * 64, 0, st>>>(a);
+                   launch_fit_frames_t<PPL, SHP, NW, false>(a, nframes, st);
                  })));
 }
 

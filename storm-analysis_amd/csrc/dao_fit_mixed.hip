@@ -8,7 +8,7 @@ namespace dao {
 void launch_fit_frames_mixed(const FitArgs& a, int nframes, hipStream_t st) {
   DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, ({
                    constexpr int NW = PPL <= 8 ? 8 : 4;
-                   k_fit_frames<PPL, SHP, NW, true><<<nframes, NW * 64, 0, st>>>(a);
+                   launch_fit_frames_t<PPL, SHP, NW, true>(a, nframes, st);
                  })));
 }
 

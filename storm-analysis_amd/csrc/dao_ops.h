@@ -1246,7 +1246,8 @@ __device__ inline bool boxes_overlap(int4 a, int4 b) {
 // frame-per-workgroup fitter
 // ---------------------------------------------------------------------------
 //
-// One workgroup of NW waves runs whole sweeps of one frame (one
+// One workgroup of nw <= NW waves (nw = blockDim.x / 64, chosen at launch)
+// runs whole sweeps of one frame (one
 // mFitIterateLM, or all of doFit, dao_fit_c.py:332-356) without returning to
 // the host.  The running peaks of a sweep are the ops 0..n-1 in index order.
 // Ops are handed out in index order, so every op below the one a wave just
@@ -1275,7 +1276,8 @@ template <int NW>
 __device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<NW>& sh) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   int count = 0;  // every thread keeps the same running total
-  for (int base = 0; base < fr.nfit; base += NW * kWave) {
+  const int nw = blockDim.x >> 6;
+  for (int base = 0; base < fr.nfit; base += nw * kWave) {
     int t = base + threadIdx.x;
     bool flag = false;
     if (t < fr.nfit) {
@@ -1296,7 +1298,7 @@ __device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<N
     __syncthreads();
     int off = count;
     int tot = 0;
-    for (int u = 0; u < NW; u++) {
+    for (int u = 0; u < nw; u++) {
       if (u < wv) off += sh.wsum[u];
       tot += sh.wsum[u];
     }
@@ -1314,7 +1316,7 @@ __device__ bool fpw_wait(const FitArgs& a, int j, int4 b, FpwShared<NW>& sh) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (long long spin = 0;; spin++) {
     bool blocked = false;
-    if (lane < NW && lane != wv) {
+    if (lane < (int)(blockDim.x >> 6) && lane != wv) {
       int c = __hip_atomic_load(&sh.cur[lane], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (c == kPending) {
         blocked = true;
@@ -1351,7 +1353,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwS
   for (int k = threadIdx.x; k < n; k += blockDim.x) a.op_done[fr.peak0 + k] = 0;
   int from = 0;
   for (;;) {
-    if (threadIdx.x < NW) sh.cur[threadIdx.x] = kIdle;
+    if (threadIdx.x < (blockDim.x >> 6)) sh.cur[threadIdx.x] = kIdle;
     if (threadIdx.x == 0) {
       sh.esc = kNoEscape;
       sh.next = from;
@@ -1423,12 +1425,20 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwS
   }
 }
 
+// Launched with nw * 64 threads (nw <= NW) and nw * kSlice doubles of
+// dynamic LDS (fit_frames_lds): fewer waves per frame leave room for more
+// workgroups per CU.
+template <int PPL, bool MP>
+constexpr int fit_frames_slice() {
+  return MP ? RoiLdsM<PPL>::kDoubles : RoiLds<PPL>::kDoubles;
+}
+
 template <int PPL, int SHAPE, int NW, bool MP>
 __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
-  constexpr int kSlice = MP ? RoiLdsM<PPL>::kDoubles : RoiLds<PPL>::kDoubles;
+  constexpr int kSlice = fit_frames_slice<PPL, MP>();
   __shared__ FpwShared<NW> sh;
-  __shared__ double lds[NW][kSlice];
-  const int wv = threadIdx.x >> 6;
+  extern __shared__ double lds_dyn[];
+  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
   const int f = a.frame_list[blockIdx.x];
   const Frame fr = a.frames[f];
   const Model& m = a.m;
@@ -1454,7 +1464,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
       a.pk.bk[p] = r;
     }
     __syncthreads();
-    fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, sh, lds[wv]);
+    fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
     const int nr = fpw_list<NW>(a, fr, SEL_RECENTER, sh);
     if (nr > 0) {
@@ -1471,7 +1481,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
         a.box[slot] = b;
       }
       __syncthreads();
-      fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_RECENTER, nr, sh, lds[wv]);
+      fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_RECENTER, nr, sh, slice);
     }
   }
   __syncthreads();
@@ -1514,6 +1524,22 @@ static int ppl_for(int nroi) {
 
 static inline dim3 waves_grid(int n) { return dim3((n + kWavesPerBlock - 1) / kWavesPerBlock); }
 static inline dim3 flat_grid(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// Launch k_fit_frames<PPL, SHP, NW, MP> for nframes frames with a.m.waves
+// waves per workgroup (0: NW; at most NW) and their LDS slices.
+template <int PPL, int SHP, int NW, bool MP>
+void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
+  const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
+  const size_t bytes = sizeof(double) * (size_t)nw * fit_frames_slice<PPL, MP>();
+  static size_t allowed = 64 * 1024;  // larger dynamic LDS must be enabled per kernel
+  if (bytes > allowed) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+      return;  // the launch below would fail; the caller's error check reports it
+    allowed = bytes;
+  }
+  k_fit_frames<PPL, SHP, NW, MP><<<nframes, nw * 64, bytes, st>>>(a);
+}
 
 // k_fit_frames launchers (dao_fit_fp64.hip, dao_fit_mixed.hip)
 void launch_fit_frames_fp64(const FitArgs& a, int nframes, hipStream_t st);

@@ -75,6 +75,7 @@ def loadBatchC():
         lib.daoBatchLoadFrames.restype = ctypes.c_int
         lib.daoBatchClearFrames.argtypes = [vp, ctypes.c_int, i32p]
         lib.daoBatchSetTiming.argtypes = [vp, ctypes.c_int]
+        lib.daoBatchSetTuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         lib.daoBatchStats.argtypes = [vp, f64p]
         _lib = lib
     return _lib
@@ -326,6 +327,10 @@ class BatchFitter(object):
         out = numpy.zeros(self.shapes[frame])
         self.clib.daoBatchGetFitImage(self.handle, frame, out)
         return out
+
+    def setTuning(self, waves=0, margin=-1):
+        """Waves per workgroup (0: the kernel's maximum) and reserved-box margin (-1: keep)."""
+        self.clib.daoBatchSetTuning(self.handle, waves, margin)
 
     def setTiming(self, on=True):
         self.clib.daoBatchSetTiming(self.handle, 1 if on else 0)

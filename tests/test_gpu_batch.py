@@ -164,3 +164,37 @@ def test_batch_device_resident_z_frames():
                                                    "xsigma": r7[:, 5], "ysigma": r7[:, 6]}, None)
         r = oracle_fit(D.MultiFitterZ, dict(kw, roi_size=18), fr, background[k].cpu().numpy())
         assert_frame_parity(out[k], r)
+
+
+@pytest.mark.gpu
+def test_tuning_keeps_results_and_autotune_picks_a_candidate():
+    """Waves per workgroup and the reserved-box margin change speed, not results."""
+    from storm_analysis_amd.sa_library import dao_tune
+    frames = scmos_frames()
+    kw = {"roi_size": 18, "sigma_range": [0.8, 6.0]}
+    bgs = [numpy.full(fr.image.shape, 100.0) for fr in frames]
+    shapes = [fr.image.shape for fr in frames]
+    caps = [max(1, len(fr.peaks["x"])) for fr in frames]
+    bf = dao_batch.BatchFitter(shapes, caps, model="3d", **kw)
+
+    def prepare():
+        for k, fr in enumerate(frames):
+            bf.setCalibration(k, rqe=fr.rqe, scmos_cal=fr.scmos_cal)
+        bf.newImages([fr.image for fr in frames])
+        bf.newBackgrounds(bgs)
+        bf.newPeaks([fr.peaks for fr in frames], "hdf5")
+
+    results = {}
+    for waves, margin in ((0, 2), (2, 1), (4, 3), (8, 2)):
+        bf.setTuning(waves, margin)
+        prepare()
+        bf.doFit()
+        results[(waves, margin)] = [bf.getPeaks(k) for k in range(len(frames))]
+    ref = results[(0, 2)]
+    for key, res in results.items():
+        for g, r in zip(res, ref):
+            assert_frame_parity(g, r)
+    best, trials = dao_tune.autotune(bf, prepare, waves=(2, 8), margins=(1, 2))
+    print("autotune trials:", trials)
+    assert best in [(t["waves"], t["margin"]) for t in trials] and len(trials) == 4
+    bf.cleanup()
