@@ -110,12 +110,15 @@ void daoBatchRemoveErrorPeaks(daoBatch *b);
 /* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
  * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */
 void daoBatchSetTiming(daoBatch *b, int on);
-/* Tuning of the frame-per-workgroup fitter: waves per workgroup (0: the
- * kernel's maximum, 8 for ROIs up to 512 pixels, else 4; fewer waves leave
- * room for more frames per CU) and the reserved-box margin in pixels (>= 0;
- * -1 keeps it).  Results are the same for every setting (bit-identical unless
- * a reserved box overflows and the rollback's arithmetic undo runs). */
-void daoBatchSetTuning(daoBatch *b, int waves, int margin);
+/* Tuning of the fitter: waves per workgroup (0: the kernel's maximum, 8 for
+ * ROIs up to 512 pixels, else 4; fewer waves leave room for more frames per
+ * CU), the reserved-box margin in pixels (>= 0; -1 keeps it) and the team
+ * size: a frame with n peaks is fitted by min(64, ceil(n / team))
+ * workgroups on as many CUs (0: one workgroup per frame; -1 keeps it), which
+ * shortens the frame's latency.  Results are the same for every setting
+ * (bit-identical unless a reserved box overflows and the rollback's
+ * arithmetic undo runs). */
+void daoBatchSetTuning(daoBatch *b, int waves, int margin, int team);
 void daoBatchStats(daoBatch *b, double *out6);
 
 #ifdef __cplusplus

@@ -144,6 +144,11 @@ void mFitRemoveErrorPeaks(fitData *fd);
 void mFitRemoveRunningPeaks(fitData *fd);
 /* mFitSetPeakStatus (multi_fit.c:1809-1831). */
 void mFitSetPeakStatus(fitData *fd, int32_t *status);
+/* No counterpart in the reference: fit the frame with a team of
+ * min(64, ceil(n_peaks / peaks_per_workgroup)) workgroups on as many CUs
+ * instead of one (0, the default: one workgroup).  Same results; shorter
+ * doFit / mFitIterateLM latency for frames with many peaks. */
+void daoSetTeam(fitData *fd, int peaks_per_workgroup);
 
 #ifdef __cplusplus
 }

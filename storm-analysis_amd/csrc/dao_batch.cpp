@@ -258,11 +258,12 @@ void daoBatchClearFrames(daoBatch* b, int n, const int* frames) {
   e.upload_frames();
 }
 
-void daoBatchSetTuning(daoBatch* b, int waves, int margin) {
+void daoBatchSetTuning(daoBatch* b, int waves, int margin, int team) {
   Engine& e = *b->eng;
   dao::Model m = e.m;
   m.waves = waves < 0 ? 0 : waves;
   if (margin >= 0) m.margin = margin;
+  if (team >= 0) m.team = team;
   e.set_model(m);
 }
 

@@ -46,6 +46,7 @@ struct Model {
   int margin;    // reserved-box margin around the ROI, pixels
   int prec;      // Precision
   int waves;     // waves per workgroup of k_fit_frames (0: the kernel's maximum)
+  int team;      // peaks per workgroup of the frame-per-team fitter (0: one workgroup per frame)
   double xoff;   // == yoff
   double zoff;
   double tol;
@@ -135,6 +136,24 @@ struct OpArgs {
   int round;
 };
 
+// Scheduling state of one frame fitted by a team of workgroups
+// (k_fit_team); only device-coherent accesses touch it during the launch.
+struct TeamCtl {
+  int phase;   // generation: raised by each transition (a new run, a rollback, the end)
+  int lead;    // the generation whose transition a workgroup has claimed
+  int kind;    // OP_LM, OP_RECENTER or kTeamDone
+  int n;       // ops of the current run
+  int next;    // next op to hand out
+  int done;    // ops of the run finished
+  int esc;     // smallest op whose LM turn left its box
+  int it;      // doFit iteration (mFitIterateLM call)
+  int sweeps;  // LM sweeps run
+  int pad[7];
+};
+constexpr int kTeamDone = -1;
+constexpr int kTeamSeats = 8;  // seats (one per wave) reserved per workgroup
+constexpr int kMaxTeam = 64;   // workgroups per frame
+
 struct FitArgs {
   Model m;
   Pix px;
@@ -150,6 +169,12 @@ struct FitArgs {
   const int* frame_list;
   int max_more;         // doFit's max_iterations; 0 = exactly one sweep
   int* err_flag;        // set when a wait exceeds its (very large) bound
+  // frame-per-team fitter only
+  TeamCtl* team_ctl;            // [F]
+  int* team_cur;                // [seats]: whether the seat's wave is inside a run
+  const int* team_block_frame;  // [workgroups]: frame of the workgroup
+  const int* team_block_seat;   // [workgroups]: first seat of the workgroup
+  const int* team_seat0;        // [F + 1]: frame f's seats are [team_seat0[f], team_seat0[f + 1])
 };
 
 }  // namespace dao

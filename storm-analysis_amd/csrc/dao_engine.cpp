@@ -142,6 +142,9 @@ Engine::~Engine() {
   dfree(d_sweeps);
   dfree(d_flist);
   dfree(d_err);
+  dfree(d_team_ctl);
+  dfree(d_team_cur);
+  dfree(d_team_tab);
   if (h_pin) (void)hipHostFree(h_pin);
   if (h_fr) (void)hipHostFree(h_fr);
   if (ev0) (void)hipEventDestroy(ev0);
@@ -376,6 +379,40 @@ int Engine::running_count() {
   return select(SEL_RUNNING);
 }
 
+// Workgroups of the frame-per-team fitter: frame f gets
+// min(kMaxTeam, ceil(nfit / m.team)) of them, the largest teams first (they
+// are dispatched first); kTeamSeats seats per workgroup.  tab = block frames,
+// block first seats, then the F + 1 seat offsets.  Returns the block count,
+// or 0 when no frame gets more than one workgroup (k_fit_frames then).
+int Engine::team_blocks(std::vector<int>& tab, int& nseats) const {
+  if (m.team <= 0) return 0;
+  std::vector<int> order, team(F, 0), s0(F + 1, 0);
+  bool multi = false;
+  int seats = 0;
+  for (int f = 0; f < F; f++) {
+    s0[f] = seats;
+    if (fr[f].nfit <= 0) continue;
+    team[f] = std::max(1, std::min(kMaxTeam, (fr[f].nfit + m.team - 1) / m.team));
+    multi = multi || team[f] > 1;
+    seats += team[f] * kTeamSeats;
+    order.push_back(f);
+  }
+  s0[F] = seats;
+  if (!multi) return 0;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return team[a] > team[b]; });
+  std::vector<int> bf, bs;
+  for (int f : order)
+    for (int k = 0; k < team[f]; k++) {
+      bf.push_back(f);
+      bs.push_back(s0[f] + k * kTeamSeats);
+    }
+  tab = bf;
+  tab.insert(tab.end(), bs.begin(), bs.end());
+  tab.insert(tab.end(), s0.begin(), s0.end());
+  nseats = seats;
+  return (int)bf.size();
+}
+
 int Engine::fit_frames(int max_more) {
   std::vector<int> list;
   for (int f = 0; f < F; f++)
@@ -387,10 +424,30 @@ int Engine::fit_frames(int max_more) {
     d_err = dalloc<int>(1);
     launch_fill_i32(d_err, 1, 0, st);
   }
-  int* h = pin(sizeof(int) * ((size_t)F + 1));
+  std::vector<int> tab;
+  int nseats = 0;
+  const int nblocks = team_blocks(tab, nseats);
+  int* h = pin(sizeof(int) * ((size_t)F + 1 + tab.size()));
   std::memcpy(h, list.data(), sizeof(int) * list.size());
   DAO_CHECK(hipMemcpyAsync(d_flist, h, sizeof(int) * list.size(), hipMemcpyHostToDevice, st));
   launch_fill_i32(d_sweeps, F, 0, st);
+  if (nblocks > 0) {
+    if (d_team_ctl == nullptr) d_team_ctl = dalloc<TeamCtl>(F);
+    if (nseats > team_seat_cap) {
+      dfree(d_team_cur);
+      team_seat_cap = std::max(nseats, 2 * team_seat_cap);
+      d_team_cur = dalloc<int>(team_seat_cap);
+    }
+    if ((int)tab.size() > team_tab_cap) {
+      dfree(d_team_tab);
+      team_tab_cap = std::max((int)tab.size(), 2 * team_tab_cap);
+      d_team_tab = dalloc<int>(team_tab_cap);
+    }
+    int* ht = h + F + 1;
+    std::memcpy(ht, tab.data(), sizeof(int) * tab.size());
+    DAO_CHECK(hipMemcpyAsync(d_team_tab, ht, sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
+    launch_team_init(d_team_ctl, F, d_team_cur, nseats, st);
+  }
   FitArgs a;
   a.m = m;
   a.px = px;
@@ -406,12 +463,21 @@ int Engine::fit_frames(int max_more) {
   a.frame_list = d_flist;
   a.max_more = max_more;
   a.err_flag = d_err;
+  a.team_ctl = d_team_ctl;
+  a.team_cur = d_team_cur;
+  a.team_block_frame = nblocks > 0 ? d_team_tab : nullptr;
+  a.team_block_seat = nblocks > 0 ? d_team_tab + nblocks : nullptr;
+  a.team_seat0 = nblocks > 0 ? d_team_tab + 2 * nblocks : nullptr;
   if (timing && ev0 == nullptr) {
     DAO_CHECK(hipEventCreate(&ev0));
     DAO_CHECK(hipEventCreate(&ev1));
   }
   if (timing) DAO_CHECK(hipEventRecord(ev0, st));
-  launch_fit_frames(a, (int)list.size(), st);
+  if (nblocks > 0) {
+    launch_fit_team(a, nblocks, st);
+  } else {
+    launch_fit_frames(a, (int)list.size(), st);
+  }
   DAO_CHECK(hipGetLastError());
   if (timing) DAO_CHECK(hipEventRecord(ev1, st));
   DAO_CHECK(hipMemcpyAsync(h, d_sweeps, sizeof(int) * F, hipMemcpyDeviceToHost, st));

@@ -48,6 +48,8 @@ void launch_select(const Peaks& pk, const Frame* frames, long long nslots, int m
 void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
 void launch_iota(int* p, int n, int base, hipStream_t st);
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st);
+void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st);
+void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st);
 void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, const Model& m, long long nslots,
                        double* out, int* status, hipStream_t st);
 
@@ -171,6 +173,12 @@ class Engine {
   int* d_sweeps = nullptr;
   int* d_flist = nullptr;
   int* d_err = nullptr;
+  // frame-per-team fitter: control words, seats, workgroup tables
+  TeamCtl* d_team_ctl = nullptr;
+  int* d_team_cur = nullptr;
+  int* d_team_tab = nullptr;  // block frame [blocks], block seat [blocks], seat0 [F + 1]
+  int team_seat_cap = 0, team_tab_cap = 0;
+  int team_blocks(std::vector<int>& tab, int& nseats) const;
   OpArgs args(int round);
 };
 

@@ -321,4 +321,10 @@ void mFitRemoveRunningPeaks(fitData* fd) {
 
 void mFitSetPeakStatus(fitData* fd, int32_t* status) { H(fd)->eng->set_status(0, status); }
 
+void daoSetTeam(fitData* fd, int peaks_per_workgroup) {
+  Handle* h = H(fd);
+  h->model.team = peaks_per_workgroup < 0 ? 0 : peaks_per_workgroup;
+  h->eng->set_model(h->model);
+}
+
 }  // extern "C"

@@ -672,6 +672,34 @@ void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, 
   k_pack_peaks<<<flat_grid(nslots), 256, 0, st>>>(pk, frames, offset, m, nslots, out, status);
 }
 
+// Team tables before a k_fit_team launch: every frame at generation 0 with
+// no escape, every seat idle.
+__global__ void k_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nframes) {
+    TeamCtl c = {};
+    c.esc = kNoEscape;
+    ctl[i] = c;
+  }
+  if (i < nseats) cur[i] = kIdle;
+}
+
+void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st) {
+  const int n = nframes > nseats ? nframes : nseats;
+  if (n > 0) k_team_init<<<flat_grid(n), 256, 0, st>>>(ctl, nframes, cur, nseats);
+}
+
+// k_fit_team over nblocks workgroups, FP64 or mixed precision (MLE only);
+// instantiations in dao_team_fp64.hip and dao_team_mixed.hip.
+void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st) {
+  if (nblocks == 0) return;
+  if (a.m.prec == PREC_MIXED && a.m.errm == EM_MLE) {
+    launch_fit_team_mixed(a, nblocks, st);
+  } else {
+    launch_fit_team_fp64(a, nblocks, st);
+  }
+}
+
 // k_fit_frames, FP64 or mixed precision (MLE only); the instantiations are
 // compiled in dao_fit_fp64.hip and dao_fit_mixed.hip.
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st) {

@@ -41,6 +41,14 @@ namespace dao {
 
 __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// A value every lane of the wave holds, in a scalar register: branches on it
+// are uniform.  Scheduler data (op index, peak index, box) are per op, hence
+// per wave; saying so keeps the LM turn's control flow uniform.
+__device__ inline int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int4 uniform_i4(int4 v) {
+  return make_int4(uniform_i(v.x), uniform_i(v.y), uniform_i(v.z), uniform_i(v.w));
+}
+
 // v of the lane selected by the DPP control CTRL (quad permutes and row
 // mirrors: every lane has a source inside its 16-lane row).
 template <int CTRL>
@@ -56,6 +64,17 @@ __device__ inline double lane_value(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
+// Lanes of one wave exchange values through LDS (the shape table): stores
+// before this point are seen by every lane's loads after it.  The wave
+// barrier alone touches no memory, so the compiler could move LDS accesses
+// across it; the wavefront-scope fences forbid that (they cost no
+// instructions: a wave's LDS accesses execute in order).
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum over the wave, the same bits in every lane: pairs, quads, 8- and
 // 16-lane rows by DPP (each step adds two equal-order partial sums, so every
 // lane of a row holds the identical row sum), then (r0 + r1) + (r2 + r3).
@@ -65,6 +84,73 @@ __device__ inline double wsum(double v) {
   v += dpp_mov<0x141>(v);  // row_half_mirror
   v += dpp_mov<0x140>(v);  // row_mirror
   return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// ---------------------------------------------------------------------------
+// access to the mutable shared arrays
+// ---------------------------------------------------------------------------
+//
+// The pixel fit state, the peak records and the op lists change during a
+// launch.  MemL1: ordinary loads and stores, coherent among the waves of one
+// workgroup (they share one CU's vector L1, which writes through).  MemDev:
+// agent-scope relaxed atomics, which go around the caches that are not
+// coherent between CUs and between XCDs, for data that workgroups on
+// different CUs share inside one launch; dev_order() then orders them.
+
+struct MemL1 {
+  template <class T>
+  __device__ static T ld(const T* p) { return *p; }
+  template <class T>
+  __device__ static void st(T* p, const T& v) { *p = v; }
+};
+
+struct MemDev {
+  template <class T>
+  __device__ static T ld(const T* p) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar access");
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  template <class T>
+  __device__ static void st(T* p, const T& v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar access");
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static int4 ld(const int4* p) {
+    const long long* q = reinterpret_cast<const long long*>(p);
+    const long long a = ld(q), b = ld(q + 1);
+    return make_int4((int)a, (int)(a >> 32), (int)b, (int)(b >> 32));
+  }
+  __device__ static void st(int4* p, const int4& v) {
+    long long* q = reinterpret_cast<long long*>(p);
+    st(q, (long long)(unsigned)v.x | ((long long)v.y << 32));
+    st(q + 1, (long long)(unsigned)v.z | ((long long)v.w << 32));
+  }
+  __device__ static PixRW ld(const PixRW* p) {
+    PixRW w;
+    w.f = ld(&p->f);
+    w.bg = ld(&p->bg);
+    w.tfi = ld(&p->tfi);
+    const long long cs = ld(reinterpret_cast<const long long*>(&p->cnt));
+    w.cnt = (int)cs;
+    w.stale = (int)(cs >> 32);
+    return w;
+  }
+  __device__ static void st(PixRW* p, const PixRW& w) {
+    st(&p->f, w.f);
+    st(&p->bg, w.bg);
+    st(&p->tfi, w.tfi);
+    st(reinterpret_cast<long long*>(&p->cnt), (long long)(unsigned)w.cnt | ((long long)w.stale << 32));
+  }
+};
+
+// Every earlier memory access of this wave has completed (a MemDev store is
+// then visible to every CU) before a later one is issued; the workgroup
+// fences keep the compiler from moving accesses across.  vmcnt(0) with
+// expcnt and lgkmcnt left open.
+__device__ inline void dev_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
 // ---------------------------------------------------------------------------
@@ -81,29 +167,31 @@ struct PeakState {
   int added;
 };
 
+template <class M = MemL1>
 __device__ inline void load_state(const PeakRec* r, PeakState& s) {
 #pragma unroll
-  for (int q = 0; q < kNFit; q++) s.prm[q] = r->prm[q];
-  s.lam = r->lam;
-  s.err = r->err;
-  s.xi = r->xi;
-  s.yi = r->yi;
-  s.status = r->status;
-  s.iter = r->iter;
-  s.added = r->added;
+  for (int q = 0; q < kNFit; q++) s.prm[q] = M::ld(&r->prm[q]);
+  s.lam = M::ld(&r->lam);
+  s.err = M::ld(&r->err);
+  s.xi = M::ld(&r->xi);
+  s.yi = M::ld(&r->yi);
+  s.status = M::ld(&r->status);
+  s.iter = M::ld(&r->iter);
+  s.added = M::ld(&r->added);
 }
 
+template <class M = MemL1>
 __device__ inline void store_state(PeakRec* r, const PeakState& s) {
   if (lane_id() == 0) {
 #pragma unroll
-    for (int q = 0; q < kNFit; q++) r->prm[q] = s.prm[q];
-    r->lam = s.lam;
-    r->err = s.err;
-    r->xi = s.xi;
-    r->yi = s.yi;
-    r->status = s.status;
-    r->iter = s.iter;
-    r->added = s.added;
+    for (int q = 0; q < kNFit; q++) M::st(&r->prm[q], s.prm[q]);
+    M::st(&r->lam, s.lam);
+    M::st(&r->err, s.err);
+    M::st(&r->xi, s.xi);
+    M::st(&r->yi, s.yi);
+    M::st(&r->status, s.status);
+    M::st(&r->iter, s.iter);
+    M::st(&r->added, s.added);
   }
 }
 
@@ -205,13 +293,14 @@ struct Roi {
     valid = false;
   }
 
+  template <class M = MemL1>
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
       if (ok(s)) {
-        const PixRW w = px.rw[k(s)];
+        const PixRW w = M::ld(px.rw + k(s));
         const PixRO r = px.ro[k(s)];
         f[s] = w.f;
         bg[s] = w.bg;
@@ -228,6 +317,7 @@ struct Roi {
     valid = true;
   }
 
+  template <class M = MemL1>
   __device__ void store(const Pix& px) {
     if (!valid) return;
 #pragma unroll
@@ -239,15 +329,16 @@ struct Roi {
         w.tfi = tfi[s];
         w.cnt = cnt[s];
         w.stale = (stale >> s) & 1u;
-        px.rw[k(s)] = w;
+        M::st(px.rw + k(s), w);
       }
     }
   }
 
+  template <class M = MemL1>
   __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi) {
     if (valid && at_x == xi && at_y == yi) return;
-    store(px);
-    load(px, fr, xi, yi);
+    store<M>(px);
+    load<M>(px, fr, xi, yi);
   }
 
   __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLds<PPL>::kTab); }
@@ -257,9 +348,9 @@ struct Roi {
   __device__ void table(const Model& m, const PeakState& st) {
     double t, e;
     lane_shape(m, st, t, e);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync_lds();
     tab()[lane_id()] = make_double2(t, e);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync_lds();
   }
 
   // psf at the held pixels for state st (cache at st.xi, st.yi); leaves the
@@ -456,7 +547,7 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
     const double v = wsum(h_acc[a]);
     if (lane_id() == 0) out[N + a] = v;
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_sync_lds();
 }
 
 // ---------------------------------------------------------------------------
@@ -566,13 +657,14 @@ struct RoiM {
     valid = false;
   }
 
+  template <class M = MemL1>
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
       if (ok(s)) {
-        const PixRW w = px.rw[k(s)];
+        const PixRW w = M::ld(px.rw + k(s));
         const PixRO r = px.ro[k(s)];
         f[s] = w.f;
         bg[s] = w.bg;
@@ -589,6 +681,7 @@ struct RoiM {
     valid = true;
   }
 
+  template <class M = MemL1>
   __device__ void store(const Pix& px) {
     if (!valid) return;
 #pragma unroll
@@ -600,23 +693,24 @@ struct RoiM {
         w.tfi = x(s) + (double)dfi[s];
         w.cnt = cnt[s];
         w.stale = (stale >> s) & 1u;
-        px.rw[k(s)] = w;
+        M::st(px.rw + k(s), w);
       }
     }
   }
 
+  template <class M = MemL1>
   __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi) {
     if (valid && at_x == xi && at_y == yi) return;
-    store(px);
-    load(px, fr, xi, yi);
+    store<M>(px);
+    load<M>(px, fr, xi, yi);
   }
 
   __device__ void table(const Model& m, const PeakState& st) {
     double t, e;
     lane_shape(m, st, t, e);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync_lds();
     tab()[lane_id()] = make_double2(t, e);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync_lds();
   }
 
   __device__ void shape(const Model& m, const PeakState& st) {
@@ -772,7 +866,7 @@ __device__ void calc_jh(const Model& m, RoiM<PPL>& c, const PeakState& st, const
     const double v = wsumf(h_acc[a]);
     if (lane_id() == 0) out[N + a] = v;
   }
-  __builtin_amdgcn_wave_barrier();
+  wave_sync_lds();
 }
 
 template <int N>
@@ -908,10 +1002,11 @@ __device__ inline int peak_check(const Model& m, const Frame& fr, PeakState& s, 
 
 // mFitResetPeak (multi_fit.c:1788-1798): back to the turn's start (still in
 // the peak's record), keeping `added`, lambda times 4.
+template <class M = MemL1>
 __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
   int added = w.added;
   double lam = w.lam;
-  load_state(r, w);
+  load_state<M>(r, w);
   w.added = added;
   w.lam = lam * 4.0;
 }
@@ -926,7 +1021,7 @@ __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
 // frame is then exactly as before the turn (the starting contribution is put
 // back) and the peak record is untouched.  d[] receives the turn's counter
 // increments.
-template <int PPL, int SHAPE, class R = Roi<PPL>>
+template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1>
 __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
                        double* slice, int p, int4 box, int* d) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
@@ -934,16 +1029,16 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   // The record keeps the turn's starting state until the turn ends, so
   // resets read it back from there instead of holding a second copy.
   PeakState w;
-  load_state(rec, w);
+  load_state<M>(rec, w);
   if (!origin_ok(m, fr, w.xi, w.yi)) {
     w.status = ST_ERROR;
-    store_state(rec, w);
+    store_state<M>(rec, w);
     return 1;
   }
 
   R c;
   c.init(m, roi_tab, fr, slice);
-  c.load(px, fr, w.xi, w.yi);
+  c.template load<M>(px, fr, w.xi, w.yi);
   c.shape(m, w);
 
   double wJ[N];
@@ -980,7 +1075,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
 
     apply_update<SHAPE>(m, w, wJ);
     if (peak_check<SHAPE>(m, fr, w, d)) {
-      reset_from(w, rec);
+      reset_from<M>(w, rec);
       w.status = ST_ERROR;
       continue;
     }
@@ -990,7 +1085,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
       break;
     }
 
-    c.move_to(px, fr, w.xi, w.yi);
+    c.template move_to<M>(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
     w.added++;
@@ -998,7 +1093,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
     if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) != 0) {
       c.stamp(w, -1);
       w.added--;
-      reset_from(w, rec);
+      reset_from<M>(w, rec);
       w.status = ST_ERROR;
       continue;
     }
@@ -1012,7 +1107,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
       d[C_NON_DECR]++;
       c.stamp(w, -1);
       w.added--;
-      reset_from(w, rec);
+      reset_from<M>(w, rec);
     } else {
       if (((start - w.err) / start) < m.tol) {
         w.status = ST_CONVERGED;
@@ -1026,37 +1121,37 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   if (escaped) {
     // Put the frame back: re-add the turn's starting contribution.  The
     // escape is detected before the candidate touches any pixel.
-    load_state(rec, w);
-    c.move_to(px, fr, w.xi, w.yi);
+    load_state<M>(rec, w);
+    c.template move_to<M>(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
-    c.store(px);
+    c.template store<M>(px);
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
     return 2;
   }
 
-  c.store(px);
-  store_state(rec, w);
+  c.template store<M>(px);
+  store_state<M>(rec, w);
   return 1;
 }
 
 // mFitRecenterPeaks body (multi_fit.c:1656-1688) for peak p, whose centre
 // left [-0.25, 1.25] of its ROI origin.
-template <int PPL, class R = Roi<PPL>>
+template <int PPL, class R = Roi<PPL>, class M = MemL1>
 __device__ void recenter_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr,
                               const int* roi_tab, double* slice, int p, int* d) {
   PeakRec* rec = pk.rec + p;
   PeakState w;
-  load_state(rec, w);
+  load_state<M>(rec, w);
   if (!origin_ok(m, fr, w.xi, w.yi)) {
     w.status = ST_ERROR;
-    store_state(rec, w);
+    store_state<M>(rec, w);
     return;
   }
   R c;
   c.init(m, roi_tab, fr, slice);
-  c.load(px, fr, w.xi, w.yi);
+  c.template load<M>(px, fr, w.xi, w.yi);
   c.shape(m, w);
   c.stamp(w, -1);
   w.added--;
@@ -1066,18 +1161,18 @@ __device__ void recenter_turn(const Model& m, const Pix& px, const Peaks& pk, co
     d[C_LOST] += 1;
     w.status = ST_ERROR;
   } else {
-    c.move_to(px, fr, w.xi, w.yi);
+    c.template move_to<M>(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
     w.added++;
   }
-  c.store(px);
-  store_state(rec, w);
+  c.template store<M>(px);
+  store_state<M>(rec, w);
 }
 
 // Add (sign +1) or remove (-1) `added` copies of state s's contribution with
 // atomic adds (undos; the callers order overlapping ones).
-template <int PPL>
+template <int PPL, class M = MemL1>
 __device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const Roi<PPL>& c,
                            const PeakState& s, int sign) {
   const int n = s.added * sign;
@@ -1095,7 +1190,7 @@ __device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const
       atomicAdd(&px.rw[k].f, (double)n * (s.prm[P_HEIGHT] * (ey * ex) * r.rqe));
       atomicAdd(&px.rw[k].bg, (double)n * (s.prm[P_BG] * r.rqe + r.scm));
       atomicAdd(&px.rw[k].cnt, n);
-      px.rw[k].stale = 1;
+      M::st(&px.rw[k].stale, 1);
     }
   }
 }
@@ -1103,17 +1198,17 @@ __device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const
 // Undo a completed LM turn of peak p: the image loses the peak's current
 // contribution and regains the one it had at the sweep's start (backup
 // record), and the peak returns to the backup.
-template <int PPL>
+template <int PPL, class M = MemL1>
 __device__ void undo_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
                           double* slice, int p) {
   PeakState w, s0;
-  load_state(pk.rec + p, w);
-  load_state(pk.bk + p, s0);
+  load_state<M>(pk.rec + p, w);
+  load_state<M>(pk.bk + p, s0);
   Roi<PPL> c;
   c.init(m, roi_tab, fr, slice);
-  undo_stamp<PPL>(m, px, fr, c, w, -1);
-  undo_stamp<PPL>(m, px, fr, c, s0, +1);
-  store_state(pk.rec + p, s0);
+  undo_stamp<PPL, M>(m, px, fr, c, w, -1);
+  undo_stamp<PPL, M>(m, px, fr, c, s0, +1);
+  store_state<M>(pk.rec + p, s0);
 }
 
 // daoNewPeaks body (dao_fit.c:1605-1751) for one new peak from a row of 4
@@ -1272,8 +1367,8 @@ struct FpwShared {
 
 // Order-preserving list of the frame's peaks that satisfy `mode`
 // (SEL_RUNNING or SEL_RECENTER) into op_peak[peak0 + k]; returns the count.
-template <int NW>
-__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<NW>& sh) {
+template <class M = MemL1>
+__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, int* wsums) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   int count = 0;  // every thread keeps the same running total
   const int nw = blockDim.x >> 6;
@@ -1282,27 +1377,27 @@ __device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, FpwShared<N
     bool flag = false;
     if (t < fr.nfit) {
       const PeakRec* r = a.pk.rec + fr.peak0 + t;
-      if (r->status == ST_RUNNING) {
+      if (M::ld(&r->status) == ST_RUNNING) {
         if (mode == SEL_RUNNING) {
           flag = true;
         } else {
-          double dx = r->prm[P_XC] - r->xi;
-          double dy = r->prm[P_YC] - r->yi;
+          double dx = M::ld(&r->prm[P_XC]) - M::ld(&r->xi);
+          double dy = M::ld(&r->prm[P_YC]) - M::ld(&r->yi);
           flag = (dx < -0.25) || (dx > 1.25) || (dy < -0.25) || (dy > 1.25);
         }
       }
     }
     unsigned long long mask = __ballot(flag);
     int pre = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) sh.wsum[wv] = __popcll(mask);
+    if (lane == 0) wsums[wv] = __popcll(mask);
     __syncthreads();
     int off = count;
     int tot = 0;
     for (int u = 0; u < nw; u++) {
-      if (u < wv) off += sh.wsum[u];
-      tot += sh.wsum[u];
+      if (u < wv) off += wsums[u];
+      tot += wsums[u];
     }
-    if (flag) a.op_peak[fr.peak0 + off + pre] = fr.peak0 + t;
+    if (flag) M::st(&a.op_peak[fr.peak0 + off + pre], fr.peak0 + t);
     count += tot;
     __syncthreads();
   }
@@ -1364,20 +1459,20 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwS
       fpw_set_cur(&sh.cur[wv], kPending);
       int j = 0;
       if (lane == 0) j = atomicAdd(&sh.next, 1);
-      j = __shfl(j, 0, kWave);
+      j = uniform_i(j);
       if (j >= n) {
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
       const int slot = fr.peak0 + j;
-      const int4 b = a.box[slot];
+      const int4 b = uniform_i4(a.box[slot]);
       if (lane == 0) sh.curbox[wv] = b;
       fpw_set_cur(&sh.cur[wv], j);
       if (!fpw_wait<NW>(a, j, b, sh)) {
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
-      const int p = a.op_peak[slot];
+      const int p = uniform_i(a.op_peak[slot]);
       int d[C_COUNT];
 #pragma unroll
       for (int q = 0; q < C_COUNT; q++) d[q] = 0;
@@ -1446,7 +1541,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   __syncthreads();
   int sweeps = 0;
   for (int it = 0;; it++) {
-    const int n = fpw_list<NW>(a, fr, SEL_RUNNING, sh);
+    const int n = fpw_list(a, fr, SEL_RUNNING, sh.wsum);
     if (it > 0 && (n == 0 || it > a.max_more)) break;
     if (n == 0) continue;  // the unconditional first sweep found nothing
     sweeps++;
@@ -1466,7 +1561,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
     __syncthreads();
     fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
-    const int nr = fpw_list<NW>(a, fr, SEL_RECENTER, sh);
+    const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum);
     if (nr > 0) {
       for (int k = threadIdx.x; k < nr; k += blockDim.x) {
         const int slot = fr.peak0 + k;
@@ -1488,6 +1583,364 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
     atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
   if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;
+}
+
+// ---------------------------------------------------------------------------
+// frame-per-team fitter
+// ---------------------------------------------------------------------------
+//
+// A team of workgroups, on any CUs, runs all of doFit for one frame, like
+// k_fit_frames does with one workgroup.  The frame's scheduling state is a
+// TeamCtl, the run's per-op boxes and finished flags, and one seat per wave
+// (inside the run or not) in global memory; every access to it and to the
+// frame's pixel fit state and peak records is a MemDev access, ordered by
+// dev_order().  The order rule is k_fit_frames': ops are handed out in
+// index order (an atomic counter) and op j starts once no earlier op that
+// has not finished overlaps j's box, so overlapping ops keep the
+// reference's update order.
+//
+// Between runs (an LM sweep, a recentering pass) one workgroup makes the
+// transition: it claims it (compare-and-swap of `lead` from the current
+// generation g to g + 1), waits until every seat is idle, builds the next
+// run (or rolls back after an escape, or ends the fit) and publishes
+// generation g + 1.  A wave marks its seat active before it checks, ahead
+// of each op it takes, that no transition out of its generation has been
+// claimed; a claimant marks `lead` first and then reads the seats.  Each
+// side's store completes before its load is issued, so at least one of them
+// sees the other: no op is handed out across a transition.  No workgroup
+// ever waits for a team mate that has not started (teams need not be
+// co-resident), and the smallest unfinished op can always run, so every
+// run ends.
+
+constexpr long long kTeamSpin = 1ll << 22;  // polls before a wait gives up (seconds)
+constexpr int kActive = 1;                  // seat of a wave inside team_ops (else kIdle)
+
+struct TeamShared {
+  int wsum[16];
+  int cnt[C_COUNT];
+  int gen;   // generation of the current action
+  int act;   // kActRun, kActLead or kActStop
+  int kind;  // the run's kind (OP_LM, OP_RECENTER, kTeamDone)
+  int n;     // the run's op count
+};
+constexpr int kActRun = 0, kActLead = 1, kActStop = 2;
+
+// Wave 0: wait for a generation other than g, or claim the transition out
+// of g once it is due (g == 0: the frame's first run; then: the run's ops
+// are all done, or one escaped).  Returns the action; sh.gen its generation.
+__device__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, TeamShared& sh) {
+  for (long long spin = 0;; spin++) {
+    const int ph = uniform_i(MemDev::ld(&ctl->phase));
+    if (ph != g) {
+      dev_order();
+      if (lane_id() == 0) sh.gen = ph;
+      return kActRun;
+    }
+    bool due = g == 0;
+    if (!due) {
+      const int esc = uniform_i(MemDev::ld(&ctl->esc));
+      const int done = uniform_i(MemDev::ld(&ctl->done));
+      const int n = uniform_i(MemDev::ld(&ctl->n));
+      due = esc != kNoEscape || done == n;
+    }
+    if (due && uniform_i(MemDev::ld(&ctl->lead)) == g) {
+      int won = 0;
+      if (lane_id() == 0) won = atomicCAS(&ctl->lead, g, g + 1) == g;
+      if (uniform_i(won)) {
+        dev_order();
+        if (lane_id() == 0) sh.gen = g;
+        return kActLead;
+      }
+    }
+    if (spin > kTeamSpin) {
+      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+      return kActStop;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// Wave 0 of the claimant: wait until no seat of the frame holds or is
+// taking an op.
+__device__ bool team_quiet(const FitArgs& a, int s0, int s1) {
+  for (long long spin = 0;; spin++) {
+    bool busy = false;
+    for (int s = s0 + lane_id(); s < s1; s += kWave) busy |= MemDev::ld(&a.team_cur[s]) != kIdle;
+    if (!__any(busy)) {
+      dev_order();
+      return true;
+    }
+    if (spin > kTeamSpin) {
+      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Wait until op j (box b) may start: no earlier op of the run that has not
+// finished overlaps b.  Every op below j has been handed out (the counter
+// hands them out in order), so the unfinished ones below j are exactly the
+// held or waiting ones.  `lo` is this wave's bound below which every op has
+// finished; it only grows during a run.  False when an earlier op escaped
+// (op j must not run now) or the wait gave up.
+__device__ bool team_wait(const FitArgs& a, TeamCtl* ctl, const Frame& fr, int j, int4 b, int& lo) {
+  for (long long spin = 0;; spin++) {
+    while (lo < j) {
+      const int i = lo + lane_id();
+      const bool open = i < j && MemDev::ld(&a.op_done[fr.peak0 + i]) == 0;
+      const unsigned long long m = __ballot(open);
+      if (m) {
+        lo += __builtin_ctzll(m);
+        break;
+      }
+      lo = min(lo + kWave, j);
+    }
+    bool blocked = false;
+    for (int i = lo + lane_id(); i < j; i += kWave) {
+      const int slot = fr.peak0 + i;
+      if (MemDev::ld(&a.op_done[slot]) == 0) blocked |= boxes_overlap(b, MemDev::ld(&a.box[slot]));
+    }
+    if (!__any(blocked)) {
+      dev_order();
+      return true;
+    }
+    if (uniform_i(MemDev::ld(&ctl->esc)) < j) return false;
+    if (spin > kTeamSpin) {
+      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// One wave takes and runs ops of generation g until none is left, a
+// transition is claimed or an op escapes.  Its seat reads kActive meanwhile.
+// Kept out of line: inlined into k_fit_team (scheduler loop, transitions and
+// LM turn in one register allocation) the fits came out wrong in two builds
+// while every out-of-line build matched the oracle, and instrumenting the
+// inlined build made the error vanish; the cause was not found.  One call
+// per run and workgroup costs nothing measurable.
+template <int PPL, int SHAPE, bool MP>
+__device__ __noinline__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
+                         TeamShared& sh, double* slice) {
+  using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
+  const int lane = lane_id();
+  if (lane == 0) MemDev::st(&a.team_cur[seat], kActive);
+  dev_order();
+  int lo = 0;
+  for (;;) {
+    if (uniform_i(MemDev::ld(&ctl->lead)) != g) break;
+    if (uniform_i(MemDev::ld(&ctl->esc)) != kNoEscape) break;
+    int j = 0;
+    if (lane == 0) j = atomicAdd(&ctl->next, 1);
+    j = uniform_i(j);
+    if (j >= n) break;
+    const int slot = fr.peak0 + j;
+    const int4 b = uniform_i4(MemDev::ld(&a.box[slot]));
+    if (!team_wait(a, ctl, fr, j, b, lo)) break;
+    const int p = uniform_i(MemDev::ld(&a.op_peak[slot]));
+    int d[C_COUNT];
+#pragma unroll
+    for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+    int r = 1;
+    if (kind == OP_LM) {
+      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d);
+    } else {
+      recenter_turn<PPL, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
+    }
+    if (r == 2) {
+      if (lane == 0) atomicMin(&ctl->esc, j);
+      break;
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < C_COUNT; q++) {
+        MemDev::st(&a.op_delta[(long long)slot * C_COUNT + q], d[q]);
+        if (d[q]) atomicAdd(&sh.cnt[q], d[q]);
+      }
+    }
+    dev_order();  // the turn's stores have completed
+    if (lane == 0) {
+      MemDev::st(&a.op_done[slot], 1);
+      atomicAdd(&ctl->done, 1);
+    }
+  }
+  dev_order();
+  if (lane == 0) MemDev::st(&a.team_cur[seat], kIdle);
+  dev_order();
+}
+
+__device__ inline void team_sync() {
+  dev_order();
+  __syncthreads();
+}
+
+// Reserved boxes (ROI at the sweep's start + margin) and backups of the LM
+// run's ops, as in k_fit_frames.
+__device__ void team_boxes_lm(const FitArgs& a, const Frame& fr, int n) {
+  const Model& m = a.m;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int slot = fr.peak0 + k;
+    const int p = MemDev::ld(&a.op_peak[slot]);
+    const PeakRec* r = a.pk.rec + p;
+    const int xi = MemDev::ld(&r->xi), yi = MemDev::ld(&r->yi);
+    int4 b;
+    b.x = max(0, xi - m.margin);
+    b.y = max(0, yi - m.margin);
+    b.z = min(fr.W, xi + m.roi + m.margin);
+    b.w = min(fr.H, yi + m.roi + m.margin);
+    MemDev::st(&a.box[slot], b);
+    const long long* src = reinterpret_cast<const long long*>(r);
+    long long* dst = reinterpret_cast<long long*>(a.pk.bk + p);
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(PeakRec) / 8); q++) MemDev::st(dst + q, MemDev::ld(src + q));
+    MemDev::st(&a.op_done[slot], 0);
+  }
+}
+
+// Boxes of the recentering ops (old and new ROI), as in k_fit_frames.
+__device__ void team_boxes_recenter(const FitArgs& a, const Frame& fr, int n) {
+  const Model& m = a.m;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int slot = fr.peak0 + k;
+    const PeakRec* r = a.pk.rec + MemDev::ld(&a.op_peak[slot]);
+    const int xi = MemDev::ld(&r->xi), yi = MemDev::ld(&r->yi);
+    const int nx = (int)floor(MemDev::ld(&r->prm[P_XC]));
+    const int ny = (int)floor(MemDev::ld(&r->prm[P_YC]));
+    int4 b;
+    b.x = max(0, min(fr.W, min(xi, nx)));
+    b.y = max(0, min(fr.H, min(yi, ny)));
+    b.z = max(b.x, min(fr.W, max(xi, nx) + m.roi));
+    b.w = max(b.y, min(fr.H, max(yi, ny) + m.roi));
+    MemDev::st(&a.box[slot], b);
+    MemDev::st(&a.op_done[slot], 0);
+  }
+}
+
+// The transition out of generation g, by the whole claimant workgroup once
+// the frame is quiet: roll back after an escape (as fpw_run), or start the
+// next run of doFit's loop (as k_fit_frames), or end the fit; then publish
+// generation g + 1.
+template <int PPL>
+__device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int f, int g, TeamShared& sh,
+                                double* slice) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const int kind = MemDev::ld(&ctl->kind);
+  const int n = MemDev::ld(&ctl->n);
+  const int esc = MemDev::ld(&ctl->esc);
+  int it = MemDev::ld(&ctl->it);
+  int sweeps = MemDev::ld(&ctl->sweeps);
+  int nk = kind, nn = n, nnext = 0, ndone = 0;
+  if (g > 0 && esc != kNoEscape) {
+    // every op below esc has finished (ops are handed out in order, and only
+    // ops above an escape give theirs up); undo the finished ones above it,
+    // newest first, and re-run from esc with the whole frame as its box
+    if (wv == 0) {
+      for (int k = n - 1; k > esc; k--) {
+        const int slot = fr.peak0 + k;
+        if (uniform_i(MemDev::ld(&a.op_done[slot]))) {
+          undo_turn<PPL, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, uniform_i(MemDev::ld(&a.op_peak[slot])));
+          dev_order();
+          if (lane == 0) {
+            for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= MemDev::ld(&a.op_delta[(long long)slot * C_COUNT + q]);
+            MemDev::st(&a.op_done[slot], 0);
+          }
+        }
+      }
+      if (lane == 0) MemDev::st(&a.box[fr.peak0 + esc], make_int4(0, 0, fr.W, fr.H));
+    }
+    nnext = esc;
+    ndone = esc;
+  } else {
+    bool lm_next = true;
+    if (g > 0 && kind == OP_LM) {
+      // mFitRecenterPeaks (multi_fit.c:1644-1690) after the sweep
+      const int nr = fpw_list<MemDev>(a, fr, SEL_RECENTER, sh.wsum);
+      if (nr > 0) {
+        team_boxes_recenter(a, fr, nr);
+        nk = OP_RECENTER;
+        nn = nr;
+        lm_next = false;
+      }
+    }
+    if (lm_next) {
+      if (g > 0) it++;
+      const int nl = fpw_list<MemDev>(a, fr, SEL_RUNNING, sh.wsum);
+      if (nl == 0 || it > a.max_more) {
+        nk = kTeamDone;
+        nn = 0;
+      } else {
+        team_boxes_lm(a, fr, nl);
+        nk = OP_LM;
+        nn = nl;
+        sweeps++;
+      }
+    }
+  }
+  team_sync();
+  if (threadIdx.x == 0) {
+    MemDev::st(&ctl->kind, nk);
+    MemDev::st(&ctl->n, nn);
+    MemDev::st(&ctl->next, nnext);
+    MemDev::st(&ctl->done, ndone);
+    MemDev::st(&ctl->esc, kNoEscape);
+    MemDev::st(&ctl->it, it);
+    MemDev::st(&ctl->sweeps, sweeps);
+    if (nk == kTeamDone) MemDev::st(&a.frame_sweeps[f], sweeps);
+    dev_order();
+    MemDev::st(&ctl->phase, g + 1);
+    dev_order();
+  }
+  __syncthreads();
+}
+
+// Workgroup b of the launch belongs to frame team_block_frame[b]; its waves
+// use seats team_block_seat[b] + wave.  Dynamic LDS as k_fit_frames.
+template <int PPL, int SHAPE, int NW, bool MP>
+__global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
+  static_assert(NW <= kTeamSeats, "seats per workgroup");
+  constexpr int kSlice = fit_frames_slice<PPL, MP>();
+  __shared__ TeamShared sh;
+  extern __shared__ double lds_dyn[];
+  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
+  const int f = a.team_block_frame[blockIdx.x];
+  const Frame fr = a.frames[f];
+  TeamCtl* const ctl = a.team_ctl + f;
+  const int s0 = a.team_seat0[f], s1 = a.team_seat0[f + 1];
+  const int seat = a.team_block_seat[blockIdx.x] + (threadIdx.x >> 6);
+  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int g = 0;  // the last generation this workgroup ran (0: none)
+  for (;;) {
+    if (threadIdx.x < kWave) {
+      const int act = team_await(a, ctl, g, sh);
+      if (threadIdx.x == 0) {
+        sh.act = act;
+        if (act == kActRun) {
+          sh.kind = MemDev::ld(&ctl->kind);
+          sh.n = MemDev::ld(&ctl->n);
+        }
+      }
+    }
+    __syncthreads();
+    const int act = sh.act, gen = sh.gen, kind = sh.kind, n = sh.n;
+    __syncthreads();
+    if (act == kActStop) break;
+    if (act == kActLead) {
+      if (threadIdx.x < kWave) team_quiet(a, s0, s1);
+      __syncthreads();
+      team_transition<PPL>(a, fr, ctl, f, gen, sh, slice);
+      continue;  // g unchanged: the next await sees the new generation
+    }
+    g = gen;
+    if (kind == kTeamDone) break;
+    team_ops<PPL, SHAPE, MP>(a, fr, ctl, g, kind, n, seat, sh, slice);
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
+    atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1541,8 +1994,26 @@ void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
   k_fit_frames<PPL, SHP, NW, MP><<<nframes, nw * 64, bytes, st>>>(a);
 }
 
-// k_fit_frames launchers (dao_fit_fp64.hip, dao_fit_mixed.hip)
+// Launch k_fit_team<PPL, SHP, NW, MP> over nblocks workgroups (the team
+// tables of `a` map them to frames and seats), waves as k_fit_frames.
+template <int PPL, int SHP, int NW, bool MP>
+void launch_fit_team_t(const FitArgs& a, int nblocks, hipStream_t st) {
+  const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
+  const size_t bytes = sizeof(double) * (size_t)nw * fit_frames_slice<PPL, MP>();
+  static size_t allowed = 64 * 1024;
+  if (bytes > allowed) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_team<PPL, SHP, NW, MP>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+      return;
+    allowed = bytes;
+  }
+  k_fit_team<PPL, SHP, NW, MP><<<nblocks, nw * 64, bytes, st>>>(a);
+}
+
+// k_fit_frames / k_fit_team launchers (dao_fit_fp64.hip, dao_fit_mixed.hip)
 void launch_fit_frames_fp64(const FitArgs& a, int nframes, hipStream_t st);
 void launch_fit_frames_mixed(const FitArgs& a, int nframes, hipStream_t st);
+void launch_fit_team_fp64(const FitArgs& a, int nblocks, hipStream_t st);
+void launch_fit_team_mixed(const FitArgs& a, int nblocks, hipStream_t st);
 
 }  // namespace dao

@@ -75,7 +75,7 @@ def loadBatchC():
         lib.daoBatchLoadFrames.restype = ctypes.c_int
         lib.daoBatchClearFrames.argtypes = [vp, ctypes.c_int, i32p]
         lib.daoBatchSetTiming.argtypes = [vp, ctypes.c_int]
-        lib.daoBatchSetTuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        lib.daoBatchSetTuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.daoBatchStats.argtypes = [vp, f64p]
         _lib = lib
     return _lib
@@ -328,9 +328,13 @@ class BatchFitter(object):
         self.clib.daoBatchGetFitImage(self.handle, frame, out)
         return out
 
-    def setTuning(self, waves=0, margin=-1):
-        """Waves per workgroup (0: the kernel's maximum) and reserved-box margin (-1: keep)."""
-        self.clib.daoBatchSetTuning(self.handle, waves, margin)
+    def setTuning(self, waves=0, margin=-1, team=-1):
+        """
+        Waves per workgroup (0: the kernel's maximum), reserved-box margin
+        (-1: keep) and team size: peaks per workgroup when several
+        workgroups share a frame (0: one workgroup per frame; -1: keep).
+        """
+        self.clib.daoBatchSetTuning(self.handle, waves, margin, team)
 
     def setTiming(self, on=True):
         self.clib.daoBatchSetTiming(self.handle, 1 if on else 0)

@@ -141,6 +141,8 @@ def setDaoFitArgtypes(lib):
         getattr(lib, name).argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.daoInitializeZ.argtypes = [vp, f64, f64, ctypes.c_double, ctypes.c_double]
     lib.daoNewPeaks.argtypes = [vp, f64, ctypes.c_char_p, ctypes.c_int]
+    if hasattr(lib, "daoSetTeam"):
+        lib.daoSetTeam.argtypes = [vp, ctypes.c_int]
     return lib
 
 
@@ -385,9 +387,10 @@ class MultiFitterGaussian(MultiFitter):
     """
     Gaussian-PSF fitters (dao_fit_c.py:560-603).  The default ROI is 10 px.
     """
-    def __init__(self, roi_size=10, wx_params=None, wy_params=None, **kwds):
+    def __init__(self, roi_size=10, wx_params=None, wy_params=None, team=0, **kwds):
         super(MultiFitterGaussian, self).__init__(**kwds)
         self.roi_size = roi_size
+        self.team = team
         self.wx_params = wx_params
         self.wy_params = wy_params
         self.clib = loadDaoFitC()
@@ -408,6 +411,18 @@ class MultiFitterGaussian(MultiFitter):
                                             self.roi_size)
         if not self.mfit:
             raise MultiFitterException("daoInitialize failed (no usable GPU, or bad sizes)")
+        if self.team:
+            self.clib.daoSetTeam(self.mfit, self.team)
+
+    def setTeam(self, peaks_per_workgroup):
+        """
+        Not in the reference: fit the frame with several workgroups, one per
+        `peaks_per_workgroup` peaks (at most 64; 0: one workgroup).  Same
+        results, lower latency for frames with many peaks.
+        """
+        self.team = peaks_per_workgroup
+        if self.mfit is not None:
+            self.clib.daoSetTeam(self.mfit, self.team)
 
     def newPeaks(self, peaks, peaks_type):
         c_peaks = self.formatPeaks(peaks, peaks_type)
