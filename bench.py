@@ -75,8 +75,14 @@ def parse():
     p.add_argument("--margin", type=int, default=2)
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
-    p.add_argument("--serve-loads", default="0.25,0.5,1.0",
+    p.add_argument("--serve-loads", default="0.05,0.2",
                    help="offered loads of the serving leg, as fractions of 16 cameras at U(50, 400) frames/s")
+    p.add_argument("--latency-frames", type=int, default=30,
+                   help="frames fitted one at a time for the single-frame latency leg (0: skip)")
+    p.add_argument("--team", type=int, default=64,
+                   help="peaks per workgroup of the frame-per-team fitter in the latency and serving legs")
+    p.add_argument("--serve-slots", type=int, default=4,
+                   help="frame slots per serving worker in the latency set-up of the serving leg")
     p.add_argument("--no-uneven", action="store_true",
                    help="skip the batched-uneven-frames leg (sCMOS 3D, 128^2..2048^2, autotuned)")
     p.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
@@ -256,9 +262,10 @@ def uneven_leg(log, seed=20261209):
     wx, wy), sigma U(1.2, 2.0), ROI 18: 8 frames of 128^2, 8 of 256^2, 4 of
     512^2 at densities U(0.1, 3) emitters/um^2 (160 nm pixels) and one 2048^2
     frame at 0.2 /um^2; per-pixel offset N(100, 5), gain N(2, 0.1), read
-    variance lognormal(ln 4, 0.8).  The autotuner picks waves per workgroup
-    and the reserved-box margin; FP64 and mixed precision are timed with the
-    pick.
+    variance lognormal(ln 4, 0.8).  The autotuner picks the reserved-box
+    margin and the team size (peaks per workgroup; 0 = one workgroup per
+    frame, which leaves the 2048^2 frame on one CU); FP64 and mixed precision
+    are timed with the pick.
     """
     import numpy as np
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D, dao_tune
@@ -286,51 +293,104 @@ def uneven_leg(log, seed=20261209):
             bf.newPeaks([f.peaks for f in frames], "hdf5")
 
         if prec == "mixed":
-            best, trials = dao_tune.autotune(bf, prepare, waves=(2, 4, 8), margins=(1, 2, 3), log=log)
-            out["autotune"] = {"best": {"waves": best[0], "margin": best[1]}, "trials": trials}
+            best, trials = dao_tune.autotune(bf, prepare, waves=(8,), margins=(1, 2), teams=(0, 500, 250, 125),
+                                             log=log)
+            out["autotune"] = {"best": {"waves": best[0], "margin": best[1], "team": best[2]}, "trials": trials}
         else:
-            bf.setTuning(out["autotune"]["best"]["waves"], out["autotune"]["best"]["margin"])
+            b = out["autotune"]["best"]
+            bf.setTuning(b["waves"], b["margin"], b["team"])
         prepare()
+        s0 = bf.stats()
         t0 = time.perf_counter()
         bf.doFit()
         rows, status, counts = bf.getAllPeaks()
         dt = time.perf_counter() - t0
         conv = int((status == D.CONVERGED).sum())
-        out[prec] = {"fit_and_readback_s": dt, "converged": conv, "fits_per_s": conv / dt}
-        log("uneven %s: %d converged in %.3f s (%.0f fits/s)" % (prec, conv, dt, conv / dt))
+        cycles = sum(bf.getCounters(k)["n_iterations"] for k in range(len(frames)))
+        sweeps = bf.stats()["sweeps"] - s0["sweeps"]
+        out[prec] = {"fit_and_readback_s": dt, "converged": conv, "fits_per_s": conv / dt,
+                     "running_at_end": int((status == D.RUNNING).sum()), "lm_cycles": int(cycles),
+                     "sweeps_of_slowest_frame": int(sweeps)}
+        log("uneven %s: %d converged in %.3f s (%.0f fits/s); %d LM cycles, %d sweeps, %d still running" %
+            (prec, conv, dt, conv / dt, cycles, sweeps, out[prec]["running_at_end"]))
         bf.cleanup()
+    return out
+
+
+def latency_leg(args, images, backgrounds, rows, counts_np, log):
+    """
+    Single-frame latency: frames of the main workload fitted one at a time,
+    inputs already in HBM, time = load + doFit + all peaks back to the host.
+    Mixed precision; one workgroup per frame, then the frame-per-team fitter
+    (one workgroup per args.team peaks, reserved-box margin 1).
+    """
+    import numpy as np
+    import torch
+    from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
+    S, N = images.shape[1], int(counts_np[0])
+    nfr = min(args.latency_frames, images.shape[0])
+    starts = np.concatenate([[0], np.cumsum(counts_np)])
+    out = {"frames": nfr, "frame_size": S, "emitters": N, "precision": "mixed", "runs": []}
+    for team, margin in ((0, 2), (args.team, 1)):
+        bf = dao_batch.BatchFitter([(S, S)], [N], model="Z", precision="mixed", roi_size=ROI, **zkw())
+        bf.setTuning(0, margin, team)
+        ts, conv = [], 0
+        for k in range(nfr + 1):
+            r = rows[int(starts[k]):int(starts[k + 1])]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            bf.loadFrames([0], images[k:k + 1], backgrounds[k:k + 1], [r], "hdf5")
+            bf.doFit()
+            _, status, _ = bf.getAllPeaks()
+            dt = (time.perf_counter() - t0) * 1e3
+            if k > 0:  # the first frame warms the kernels
+                ts.append(dt)
+                conv += int((status == D.CONVERGED).sum())
+        bf.cleanup()
+        ts = np.array(ts)
+        run = {"team": team, "margin": margin, "p50_ms": float(np.percentile(ts, 50)),
+               "p99_ms": float(np.percentile(ts, 99)), "max_ms": float(ts.max()), "converged": conv}
+        out["runs"].append(run)
+        log("latency team %d margin %d: p50 %.1f ms, p99 %.1f ms per %dx%d frame" %
+            (team, margin, run["p50_ms"], run["p99_ms"], S, S))
     return out
 
 
 def serve_leg(args, precision, log):
     """
-    Continuous-batching server (8 workers x 32 slots of 1024^2 / 4000 peaks)
+    Continuous-batching server (8 workers, slots of 1024^2 / 4000 peaks)
     under 16 simulated cameras with Poisson arrivals, sensors of 256^2,
     512^2 or 1024^2 and ~1000 emitters per 512^2, Z model; p50/p99 latency
-    and fits/s per offered load.
+    and fits/s per offered load.  Two set-ups: throughput (32 slots per
+    worker, one workgroup per frame) and latency (args.serve_slots slots,
+    frame-per-team fitter with one workgroup per args.team peaks).
     """
     from storm_analysis_amd.sa_library import dao_serve
     from storm_analysis_amd.diagnostics import serve_sim
     pools = serve_sim.make_pools((256, 512, 1024))
-    srv_kw = dict(workers=8, slots=32, max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI,
-                  precision=precision, **zkw())
-    out = {"server": {k: v for k, v in srv_kw.items() if k in ("workers", "slots", "max_shape", "max_peaks",
-                                                               "precision")},
+    setups = {"throughput": dict(slots=32, team=0, margin=2), "latency": dict(slots=args.serve_slots, team=args.team,
+                                                                             margin=1)}
+    base = dict(workers=8, max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI, precision=precision,
+                **zkw())
+    out = {"server": {"workers": 8, "max_shape": [1024, 1024], "max_peaks": 4000, "precision": precision,
+                      "setups": setups},
            "cameras": 16, "rates_fps": [50, 400], "sizes": [256, 512, 1024], "runs": []}
-    # warm the kernels once
-    srv = dao_serve.FitServer(**srv_kw)
-    serve_sim.run(srv, duration_s=0.2, load=0.1, pools=pools)
-    srv.close()
-    for load in [float(x) for x in args.serve_loads.split(",") if x]:
-        srv = dao_serve.FitServer(**srv_kw)
-        try:
-            summ = serve_sim.run(srv, duration_s=args.serve_seconds, load=load, pools=pools)
-        finally:
-            srv.close()
-        log("serving load %.2f: %d frames, p50 %.1f ms, p99 %.1f ms, %.0f fits/s (offered %.0f emitters/s)" %
-            (load, summ["requests"], summ["p50_ms"], summ["p99_ms"], summ["fits_per_s"],
-             summ["offered_emitters_per_s"]))
-        out["runs"].append(summ)
+    for name, su in setups.items():
+        srv_kw = dict(base, **su)
+        srv = dao_serve.FitServer(**srv_kw)  # warm the kernels once
+        serve_sim.run(srv, duration_s=0.2, load=0.05, pools=pools)
+        srv.close()
+        for load in [float(x) for x in args.serve_loads.split(",") if x]:
+            srv = dao_serve.FitServer(**srv_kw)
+            try:
+                summ = serve_sim.run(srv, duration_s=args.serve_seconds, load=load, pools=pools)
+            finally:
+                srv.close()
+            summ["setup"] = name
+            log("serving %s load %.2f: %d frames, p50 %.1f ms, p99 %.1f ms, %.0f fits/s (offered %.0f emitters/s)"
+                % (name, load, summ["requests"], summ["p50_ms"], summ["p99_ms"], summ["fits_per_s"],
+                   summ["offered_emitters_per_s"]))
+            out["runs"].append(summ)
     return out
 
 
@@ -407,6 +467,10 @@ def main():
             n = int(numpy.sum(per_frame[:nfr]))
             accuracy["%s_vs_oracle" % p] = agreement((out_p[:n], st_p[:n]), (orow, ost), D)
 
+    latency = None
+    if args.latency_frames > 0:
+        latency = latency_leg(args, images, backgrounds, rows, counts_np, log)
+
     uneven = None
     if not args.no_uneven:
         uneven = uneven_leg(log)
@@ -441,6 +505,7 @@ def main():
                   "path": "MultiFitterZ, one frame per call, FP64"},
         "batched_vs_plain": hm["fits_per_s"] / plain_fps if plain_fps > 0 else None,
         "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
+        "single_frame_latency": latency,
         "accuracy": accuracy,
         "mixed_xy_bar_px": TOL_MIXED_XY_PX,
         "uneven_batch": uneven,

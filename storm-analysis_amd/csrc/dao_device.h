@@ -148,7 +148,8 @@ struct TeamCtl {
   int esc;     // smallest op whose LM turn left its box
   int it;      // doFit iteration (mFitIterateLM call)
   int sweeps;  // LM sweeps run
-  int pad[7];
+  int nrun;    // RUNNING peaks at the current sweep's start (the LM run's list)
+  int pad[6];
 };
 constexpr int kTeamDone = -1;
 constexpr int kTeamSeats = 8;  // seats (one per wave) reserved per workgroup
@@ -160,7 +161,8 @@ struct FitArgs {
   Peaks pk;
   const Frame* frames;
   const int* roi_tab;
-  int* op_peak;         // per-slot scratch: op list of the frame
+  int* op_peak;         // per-slot scratch: op list of the frame (recentering runs)
+  int* run_list;        // per-slot scratch: RUNNING peaks of the sweep (LM runs)
   int4* box;            // per-slot scratch: op boxes
   int* op_delta;        // per-slot scratch: counter increments of an op
   int* op_done;         // per-slot scratch: op finished in the current phase

@@ -181,6 +181,7 @@ void Engine::alloc_sched(long long c) {
   sc.esc_box = dalloc<int4>(c);
   d_override = dalloc<int4>(c);
   d_flag = dalloc<int>(c);
+  d_run = dalloc<int>(c);
 }
 
 void Engine::free_sched() {
@@ -196,6 +197,7 @@ void Engine::free_sched() {
   dfree(sc.esc_box);
   dfree(d_override);
   dfree(d_flag);
+  dfree(d_run);
 }
 
 // Grow every per-slot array to c slots, keeping the first `cap` records.
@@ -455,6 +457,7 @@ int Engine::fit_frames(int max_more) {
   a.frames = d_fr;
   a.roi_tab = d_roi;
   a.op_peak = sc.op_peak;
+  a.run_list = d_run;
   a.box = sc.box;
   a.op_delta = sc.op_delta;
   a.op_done = sc.op_flags;

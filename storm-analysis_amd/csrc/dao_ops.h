@@ -1366,17 +1366,27 @@ struct FpwShared {
 };
 
 // Order-preserving list of the frame's peaks that satisfy `mode`
-// (SEL_RUNNING or SEL_RECENTER) into op_peak[peak0 + k]; returns the count.
+// (SEL_RUNNING or SEL_RECENTER) into dst[peak0 + k]; returns the count.
+// The candidates are src[peak0 + k], k < nsrc (a list in index order), or,
+// with src == nullptr, every peak of the frame.  dst may be src: entry k is
+// read before any entry at or above the count of earlier matches is written.
+// Within doFit a peak that is not RUNNING never becomes RUNNING again, so a
+// sweep's RUNNING list is the previous one's filtered, and a sweep costs
+// O(RUNNING peaks), not O(all peaks), in the long tail of a fit.
 template <class M = MemL1>
-__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, int* wsums) {
+__device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, int* wsums, const int* src, int nsrc,
+                        int* dst) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const int total = src ? nsrc : fr.nfit;
   int count = 0;  // every thread keeps the same running total
   const int nw = blockDim.x >> 6;
-  for (int base = 0; base < fr.nfit; base += nw * kWave) {
+  for (int base = 0; base < total; base += nw * kWave) {
     int t = base + threadIdx.x;
     bool flag = false;
-    if (t < fr.nfit) {
-      const PeakRec* r = a.pk.rec + fr.peak0 + t;
+    int p = 0;
+    if (t < total) {
+      p = src ? M::ld(&src[fr.peak0 + t]) : fr.peak0 + t;
+      const PeakRec* r = a.pk.rec + p;
       if (M::ld(&r->status) == ST_RUNNING) {
         if (mode == SEL_RUNNING) {
           flag = true;
@@ -1397,7 +1407,7 @@ __device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, int* wsums)
       if (u < wv) off += wsums[u];
       tot += wsums[u];
     }
-    if (flag) M::st(&a.op_peak[fr.peak0 + off + pre], fr.peak0 + t);
+    if (flag) M::st(&dst[fr.peak0 + off + pre], p);
     count += tot;
     __syncthreads();
   }
@@ -1436,13 +1446,14 @@ __device__ inline void fpw_set_cur(int* cur, int v) {
   if (lane_id() == 0) __hip_atomic_store(cur, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Run ops 0..n-1 (kind OP_LM or OP_RECENTER) of the frame.  If an LM turn
+// Run ops 0..n-1 (kind OP_LM or OP_RECENTER; op k is peak list[peak0 + k]) of the frame.  If an LM turn
 // leaves its box, every op below it has finished (ops are handed out in order
 // and never wait on later ones): the finished ops above it are undone newest
 // first, the turn gets the whole frame as its box (it then runs alone and
 // cannot escape again), and the run resumes from it.
 template <int PPL, int SHAPE, int NW, bool MP>
-__device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwShared<NW>& sh, double* slice) {
+__device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, const int* list, FpwShared<NW>& sh,
+                        double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < n; k += blockDim.x) a.op_done[fr.peak0 + k] = 0;
@@ -1472,7 +1483,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwS
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
-      const int p = uniform_i(a.op_peak[slot]);
+      const int p = uniform_i(list[slot]);
       int d[C_COUNT];
 #pragma unroll
       for (int q = 0; q < C_COUNT; q++) d[q] = 0;
@@ -1505,7 +1516,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, FpwS
       for (int k = n - 1; k > e; k--) {
         const int slot = fr.peak0 + k;
         if (a.op_done[slot]) {
-          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, a.op_peak[slot]);
+          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, list[slot]);
           if (lane == 0) {
             for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= a.op_delta[(long long)slot * C_COUNT + q];
             a.op_done[slot] = 0;
@@ -1540,15 +1551,17 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
   __syncthreads();
   int sweeps = 0;
+  int n = 0;  // RUNNING peaks at the sweep's start: a.run_list[peak0 .. peak0 + n)
   for (int it = 0;; it++) {
-    const int n = fpw_list(a, fr, SEL_RUNNING, sh.wsum);
+    n = it == 0 ? fpw_list(a, fr, SEL_RUNNING, sh.wsum, nullptr, 0, a.run_list)
+                : fpw_list(a, fr, SEL_RUNNING, sh.wsum, a.run_list, n, a.run_list);
     if (it > 0 && (n == 0 || it > a.max_more)) break;
     if (n == 0) continue;  // the unconditional first sweep found nothing
     sweeps++;
     // reserved boxes (ROI at the sweep's start + margin) and backups
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
       const int slot = fr.peak0 + k;
-      const int p = a.op_peak[slot];
+      const int p = a.run_list[slot];
       const PeakRec r = a.pk.rec[p];
       int4 b;
       b.x = max(0, r.xi - m.margin);
@@ -1559,9 +1572,9 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
       a.pk.bk[p] = r;
     }
     __syncthreads();
-    fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, sh, slice);
+    fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, a.run_list, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
-    const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum);
+    const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum, a.run_list, n, a.op_peak);
     if (nr > 0) {
       for (int k = threadIdx.x; k < nr; k += blockDim.x) {
         const int slot = fr.peak0 + k;
@@ -1576,7 +1589,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
         a.box[slot] = b;
       }
       __syncthreads();
-      fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_RECENTER, nr, sh, slice);
+      fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
     }
   }
   __syncthreads();
@@ -1739,7 +1752,7 @@ __device__ __noinline__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl
     const int slot = fr.peak0 + j;
     const int4 b = uniform_i4(MemDev::ld(&a.box[slot]));
     if (!team_wait(a, ctl, fr, j, b, lo)) break;
-    const int p = uniform_i(MemDev::ld(&a.op_peak[slot]));
+    const int p = uniform_i(MemDev::ld(&(kind == OP_LM ? a.run_list : a.op_peak)[slot]));
     int d[C_COUNT];
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
@@ -1782,7 +1795,7 @@ __device__ void team_boxes_lm(const FitArgs& a, const Frame& fr, int n) {
   const Model& m = a.m;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     const int slot = fr.peak0 + k;
-    const int p = MemDev::ld(&a.op_peak[slot]);
+    const int p = MemDev::ld(&a.run_list[slot]);
     const PeakRec* r = a.pk.rec + p;
     const int xi = MemDev::ld(&r->xi), yi = MemDev::ld(&r->yi);
     int4 b;
@@ -1831,6 +1844,7 @@ __device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl,
   const int esc = MemDev::ld(&ctl->esc);
   int it = MemDev::ld(&ctl->it);
   int sweeps = MemDev::ld(&ctl->sweeps);
+  int nrun = MemDev::ld(&ctl->nrun);
   int nk = kind, nn = n, nnext = 0, ndone = 0;
   if (g > 0 && esc != kNoEscape) {
     // every op below esc has finished (ops are handed out in order, and only
@@ -1840,7 +1854,7 @@ __device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl,
       for (int k = n - 1; k > esc; k--) {
         const int slot = fr.peak0 + k;
         if (uniform_i(MemDev::ld(&a.op_done[slot]))) {
-          undo_turn<PPL, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, uniform_i(MemDev::ld(&a.op_peak[slot])));
+          undo_turn<PPL, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, uniform_i(MemDev::ld(&a.run_list[slot])));
           dev_order();
           if (lane == 0) {
             for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= MemDev::ld(&a.op_delta[(long long)slot * C_COUNT + q]);
@@ -1856,7 +1870,7 @@ __device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl,
     bool lm_next = true;
     if (g > 0 && kind == OP_LM) {
       // mFitRecenterPeaks (multi_fit.c:1644-1690) after the sweep
-      const int nr = fpw_list<MemDev>(a, fr, SEL_RECENTER, sh.wsum);
+      const int nr = fpw_list<MemDev>(a, fr, SEL_RECENTER, sh.wsum, a.run_list, nrun, a.op_peak);
       if (nr > 0) {
         team_boxes_recenter(a, fr, nr);
         nk = OP_RECENTER;
@@ -1866,7 +1880,9 @@ __device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl,
     }
     if (lm_next) {
       if (g > 0) it++;
-      const int nl = fpw_list<MemDev>(a, fr, SEL_RUNNING, sh.wsum);
+      const int nl = g == 0 ? fpw_list<MemDev>(a, fr, SEL_RUNNING, sh.wsum, nullptr, 0, a.run_list)
+                            : fpw_list<MemDev>(a, fr, SEL_RUNNING, sh.wsum, a.run_list, nrun, a.run_list);
+      nrun = nl;
       if (nl == 0 || it > a.max_more) {
         nk = kTeamDone;
         nn = 0;
@@ -1887,6 +1903,7 @@ __device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl,
     MemDev::st(&ctl->esc, kNoEscape);
     MemDev::st(&ctl->it, it);
     MemDev::st(&ctl->sweeps, sweeps);
+    MemDev::st(&ctl->nrun, nrun);
     if (nk == kTeamDone) MemDev::st(&a.frame_sweeps[f], sweeps);
     dev_order();
     MemDev::st(&ctl->phase, g + 1);

@@ -86,11 +86,13 @@ class FitServer(object):
     """
     workers x slots frame slots of max_shape (height, width) and max_peaks
     peaks each.  hbm_budget_gb, when given, caps workers x slots to what the
-    budget holds.  The remaining keywords are BatchFitter's (model,
-    precision, roi_size, wx_params, ...).
+    budget holds.  team > 0 fits each frame with one workgroup per `team`
+    peaks (lower latency per frame), margin >= 0 sets the reserved-box
+    margin.  The remaining keywords are BatchFitter's (model, precision,
+    roi_size, wx_params, ...).
     """
     def __init__(self, workers=4, slots=32, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None,
-                 max_iterations=2000, **fitter_kw):
+                 max_iterations=2000, team=0, margin=-1, **fitter_kw):
         self.max_shape = tuple(max_shape)
         self.max_peaks = max_peaks
         self.max_iterations = max_iterations
@@ -100,6 +102,8 @@ class FitServer(object):
         self.slot_bytes = slot_bytes(max_shape, max_peaks)
         self.fitters = [dao_batch.BatchFitter([self.max_shape] * slots, [max_peaks] * slots, **fitter_kw)
                         for _ in range(workers)]
+        for bf in self.fitters:
+            bf.setTuning(0, margin, team)
         self.queue = queue.Queue()
         self.completed = []
         self._lock = threading.Lock()

@@ -168,7 +168,7 @@ def test_batch_device_resident_z_frames():
 
 @pytest.mark.gpu
 def test_tuning_keeps_results_and_autotune_picks_a_candidate():
-    """Waves per workgroup and the reserved-box margin change speed, not results."""
+    """Waves per workgroup, the reserved-box margin and the team size change speed, not results."""
     from storm_analysis_amd.sa_library import dao_tune
     frames = scmos_frames()
     kw = {"roi_size": 18, "sigma_range": [0.8, 6.0]}
@@ -185,16 +185,16 @@ def test_tuning_keeps_results_and_autotune_picks_a_candidate():
         bf.newPeaks([fr.peaks for fr in frames], "hdf5")
 
     results = {}
-    for waves, margin in ((0, 2), (2, 1), (4, 3), (8, 2)):
-        bf.setTuning(waves, margin)
+    for waves, margin, team in ((0, 2, 0), (2, 1, 0), (4, 3, 0), (8, 2, 0), (4, 1, 30), (8, 2, 60)):
+        bf.setTuning(waves, margin, team)
         prepare()
         bf.doFit()
-        results[(waves, margin)] = [bf.getPeaks(k) for k in range(len(frames))]
-    ref = results[(0, 2)]
+        results[(waves, margin, team)] = [bf.getPeaks(k) for k in range(len(frames))]
+    ref = results[(0, 2, 0)]
     for key, res in results.items():
         for g, r in zip(res, ref):
             assert_frame_parity(g, r)
-    best, trials = dao_tune.autotune(bf, prepare, waves=(2, 8), margins=(1, 2))
+    best, trials = dao_tune.autotune(bf, prepare, waves=(2, 8), margins=(1, 2), teams=(0, 40))
     print("autotune trials:", trials)
-    assert best in [(t["waves"], t["margin"]) for t in trials] and len(trials) == 4
+    assert best in [(t["waves"], t["margin"], t["team"]) for t in trials] and len(trials) == 8
     bf.cleanup()
