@@ -127,3 +127,43 @@ def test_finder_start_and_removal():
     kw = {"roi_size": 18, "sigma_range": [0.75, 7.5]}
     g, r = both(D.MultiFitter2D, kw, fr.image, peaks, peaks_type="finder", remove_error=True)
     assert_parity(g, r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("frame", [315, 318])
+def test_z_headline_frames_deviations_are_within_the_oracles_own_conditioning(frame):
+    """
+    Two frames of the benchmark workload (512 x 512, 1000 emitters, Z model,
+    the bench's seeded generator) that hold the only peaks of its 500 frames
+    where the FP64 GPU fit and the oracle differ by more than the parity bar:
+    pairs of emitters closer than a pixel.  Every other peak meets the bar.
+    For the deviating ones, the oracle run on the same image with half of its
+    pixels moved by one ulp moves those peaks by as much as the GPU result
+    differs: the fit there is decided by last-bit rounding (exp() and
+    reduction order differ between the GPU and the host's libm), not by the
+    algorithm.
+    """
+    import torch
+    from storm_analysis_amd.diagnostics import synth_gpu
+    images, bgs, rows, counts, _ = synth_gpu.frames_z(frame + 1, size=512, n=1000, device="cuda:0")
+    torch.cuda.synchronize()
+    img = images[frame].cpu().numpy()
+    bg = bgs[frame].cpu().numpy()
+    r7 = rows[frame * 1000:(frame + 1) * 1000].cpu().numpy()
+    peaks = {k: r7[:, i] for i, k in enumerate(("x", "y", "z", "background", "height", "xsigma", "ysigma"))}
+    kw = {"roi_size": 18, "wx_params": synth.WX_Z, "wy_params": synth.WY_Z, "min_z": -0.5, "max_z": 0.5}
+    g, r = both(D.MultiFitterZ, kw, img, peaks, background=bg)
+    numpy.testing.assert_array_equal(g["status"], r["status"])
+    d = numpy.hypot(g["x"] - r["x"], g["y"] - r["y"])
+    off = numpy.nonzero(d > TOL_POS)[0]
+    assert 0 < off.size <= 3, off
+    spread = numpy.zeros(off.size)
+    rng = numpy.random.default_rng(1)
+    for t in range(4):
+        p = img.copy()
+        m = rng.random(p.shape) < 0.5
+        p[m] = numpy.nextafter(p[m], numpy.inf if t % 2 == 0 else -numpy.inf)
+        r2 = fit_frame("oracle", D.MultiFitterZ, kw, p, peaks, background=bg)
+        spread = numpy.maximum(spread, numpy.hypot(r2["x"] - r["x"], r2["y"] - r["y"])[off])
+    print("deviating peaks", off.tolist(), "gpu-oracle", d[off].round(4).tolist(), "oracle spread", spread.round(4).tolist())
+    assert numpy.all(spread >= 0.5 * d[off]), (d[off], spread)
