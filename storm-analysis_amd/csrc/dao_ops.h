@@ -1638,10 +1638,18 @@ struct TeamShared {
 };
 constexpr int kActRun = 0, kActLead = 1, kActStop = 2;
 
+// The scheduler's cold paths (team_await, team_quiet, team_transition) are
+// kept out of line.  With them and the LM turn inlined into k_fit_team in
+// one register allocation, the fits came out wrong in two builds while every
+// build with them out of line matched the oracle, and instrumenting the
+// inlined build made the error vanish; the cause was not found.  Out of line
+// they also leave the hot loop (take, wait, turn) fewer registers to spill:
+// a team turn got ~12% faster than with the whole op loop out of line.
+
 // Wave 0: wait for a generation other than g, or claim the transition out
 // of g once it is due (g == 0: the frame's first run; then: the run's ops
 // are all done, or one escaped).  Returns the action; sh.gen its generation.
-__device__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, TeamShared& sh) {
+__device__ __noinline__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, TeamShared& sh) {
   for (long long spin = 0;; spin++) {
     const int ph = uniform_i(MemDev::ld(&ctl->phase));
     if (ph != g) {
@@ -1675,7 +1683,7 @@ __device__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, TeamShared& sh)
 
 // Wave 0 of the claimant: wait until no seat of the frame holds or is
 // taking an op.
-__device__ bool team_quiet(const FitArgs& a, int s0, int s1) {
+__device__ __noinline__ bool team_quiet(const FitArgs& a, int s0, int s1) {
   for (long long spin = 0;; spin++) {
     bool busy = false;
     for (int s = s0 + lane_id(); s < s1; s += kWave) busy |= MemDev::ld(&a.team_cur[s]) != kIdle;
@@ -1729,13 +1737,8 @@ __device__ bool team_wait(const FitArgs& a, TeamCtl* ctl, const Frame& fr, int j
 
 // One wave takes and runs ops of generation g until none is left, a
 // transition is claimed or an op escapes.  Its seat reads kActive meanwhile.
-// Kept out of line: inlined into k_fit_team (scheduler loop, transitions and
-// LM turn in one register allocation) the fits came out wrong in two builds
-// while every out-of-line build matched the oracle, and instrumenting the
-// inlined build made the error vanish; the cause was not found.  One call
-// per run and workgroup costs nothing measurable.
 template <int PPL, int SHAPE, bool MP>
-__device__ __noinline__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
+__device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
                          TeamShared& sh, double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id();
@@ -1836,7 +1839,7 @@ __device__ void team_boxes_recenter(const FitArgs& a, const Frame& fr, int n) {
 // next run of doFit's loop (as k_fit_frames), or end the fit; then publish
 // generation g + 1.
 template <int PPL>
-__device__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int f, int g, TeamShared& sh,
+__device__ __noinline__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int f, int g, TeamShared& sh,
                                 double* slice) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int kind = MemDev::ld(&ctl->kind);
