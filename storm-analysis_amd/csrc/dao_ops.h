@@ -97,6 +97,18 @@ __device__ inline double wsum(double v) {
 // coherent between CUs and between XCDs, for data that workgroups on
 // different CUs share inside one launch; dev_order() then orders them.
 
+// All of the fitter's arrays are in global memory.  Pointers read out of the
+// kernel's argument struct are generic to the compiler once that struct's
+// address reaches a function that is not inlined, and generic accesses
+// compile to flat instructions, whose completion every LDS wait (lgkmcnt)
+// then also waits for.  gptr() states the address space.
+template <class T>
+using GlobalT = __attribute__((address_space(1))) T;
+template <class T>
+__device__ inline const GlobalT<T>* gptr(const T* p) { return (const GlobalT<T>*)p; }
+template <class T>
+__device__ inline GlobalT<T>* gptr(T* p) { return (GlobalT<T>*)p; }
+
 struct MemL1 {
   template <class T>
   __device__ static T ld(const T* p) { return *p; }
@@ -108,12 +120,24 @@ struct MemDev {
   template <class T>
   __device__ static T ld(const T* p) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar access");
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gptr(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   template <class T>
   __device__ static void st(T* p, const T& v) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar access");
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // atomic read-modify-writes of the team's control words
+  __device__ static int add(int* p, int v) {
+    return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static void min(int* p, int v) {
+    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ static int cas(int* p, int expected, int desired) {
+    __hip_atomic_compare_exchange_strong(gptr(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expected;
   }
   __device__ static int4 ld(const int4* p) {
     const long long* q = reinterpret_cast<const long long*>(p);
@@ -283,7 +307,7 @@ struct Roi {
     for (int s = 0; s < PPL; s++) {
       int j = l + s * kWave;
       bool o = j < m.nroi;
-      int v = o ? roi_tab[j] : 0;
+      int v = o ? *gptr(roi_tab + j) : 0;
       int cx = v & 0xffff, cy = v >> 16;
       rxy[s] = cx | (cy << 8);
       koff[s] = cy * fr.W + cx;
@@ -301,7 +325,8 @@ struct Roi {
     for (int s = 0; s < PPL; s++) {
       if (ok(s)) {
         const PixRW w = M::ld(px.rw + k(s));
-        const PixRO r = px.ro[k(s)];
+        const GlobalT<PixRO>* ro = gptr(px.ro + k(s));
+        const PixRO r = {ro->x, ro->rqe, ro->scm, 0.0};
         f[s] = w.f;
         bg[s] = w.bg;
         tfi[s] = w.tfi;
@@ -647,7 +672,7 @@ struct RoiM {
     for (int s = 0; s < PPL; s++) {
       int j = l + s * kWave;
       bool o = j < m.nroi;
-      int v = o ? roi_tab[j] : 0;
+      int v = o ? *gptr(roi_tab + j) : 0;
       int cx = v & 0xffff, cy = v >> 16;
       rxy[s] = cx | (cy << 8);
       koff[s] = cy * fr.W + cx;
@@ -665,7 +690,8 @@ struct RoiM {
     for (int s = 0; s < PPL; s++) {
       if (ok(s)) {
         const PixRW w = M::ld(px.rw + k(s));
-        const PixRO r = px.ro[k(s)];
+        const GlobalT<PixRO>* ro = gptr(px.ro + k(s));
+        const PixRO r = {ro->x, ro->rqe, ro->scm, 0.0};
         f[s] = w.f;
         bg[s] = w.bg;
         dfi[s] = (float)(w.tfi - r.x);
@@ -1666,7 +1692,7 @@ __device__ __noinline__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, Te
     }
     if (due && uniform_i(MemDev::ld(&ctl->lead)) == g) {
       int won = 0;
-      if (lane_id() == 0) won = atomicCAS(&ctl->lead, g, g + 1) == g;
+      if (lane_id() == 0) won = MemDev::cas(&ctl->lead, g, g + 1) == g;
       if (uniform_i(won)) {
         dev_order();
         if (lane_id() == 0) sh.gen = g;
@@ -1749,7 +1775,7 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
     if (uniform_i(MemDev::ld(&ctl->lead)) != g) break;
     if (uniform_i(MemDev::ld(&ctl->esc)) != kNoEscape) break;
     int j = 0;
-    if (lane == 0) j = atomicAdd(&ctl->next, 1);
+    if (lane == 0) j = MemDev::add(&ctl->next, 1);
     j = uniform_i(j);
     if (j >= n) break;
     const int slot = fr.peak0 + j;
@@ -1766,7 +1792,7 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
       recenter_turn<PPL, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
     }
     if (r == 2) {
-      if (lane == 0) atomicMin(&ctl->esc, j);
+      if (lane == 0) MemDev::min(&ctl->esc, j);
       break;
     }
     if (lane == 0) {
@@ -1779,7 +1805,7 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
     dev_order();  // the turn's stores have completed
     if (lane == 0) {
       MemDev::st(&a.op_done[slot], 1);
-      atomicAdd(&ctl->done, 1);
+      MemDev::add(&ctl->done, 1);
     }
   }
   dev_order();
