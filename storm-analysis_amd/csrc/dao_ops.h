@@ -27,6 +27,7 @@
 // driver then re-runs it in order (see fpw_run and Engine::sweep).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include <type_traits>
@@ -204,6 +205,37 @@ __device__ inline void load_state(const PeakRec* r, PeakState& s) {
   s.added = M::ld(&r->added);
 }
 
+// The device-coherent forms move the 96-byte record as twelve 8-byte words,
+// one per lane, in one instruction (each coherent access is its own request
+// to the memory side), and broadcast them with readlane.
+constexpr int kRecWords = (int)(sizeof(PeakRec) / 8);
+static_assert(sizeof(PeakRec) == 96 && offsetof(PeakRec, lam) == 56 && offsetof(PeakRec, err) == 64 &&
+                  offsetof(PeakRec, xi) == 72 && offsetof(PeakRec, yi) == 76 && offsetof(PeakRec, status) == 80 &&
+                  offsetof(PeakRec, iter) == 84 && offsetof(PeakRec, added) == 88,
+              "PeakRec word layout of load_state / store_state<MemDev>");
+
+__device__ inline long long lane_word(long long v, int lane) {
+  const int lo = __builtin_amdgcn_readlane((int)v, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+template <>
+__device__ inline void load_state<MemDev>(const PeakRec* r, PeakState& s) {
+  const int l = lane_id();
+  const long long v = l < kRecWords ? MemDev::ld(reinterpret_cast<const long long*>(r) + l) : 0;
+#pragma unroll
+  for (int q = 0; q < kNFit; q++) s.prm[q] = __longlong_as_double(lane_word(v, q));
+  s.lam = __longlong_as_double(lane_word(v, 7));
+  s.err = __longlong_as_double(lane_word(v, 8));
+  const long long w9 = lane_word(v, 9), w10 = lane_word(v, 10), w11 = lane_word(v, 11);
+  s.xi = (int)w9;
+  s.yi = (int)(w9 >> 32);
+  s.status = (int)w10;
+  s.iter = (int)(w10 >> 32);
+  s.added = (int)w11;
+}
+
 template <class M = MemL1>
 __device__ inline void store_state(PeakRec* r, const PeakState& s) {
   if (lane_id() == 0) {
@@ -216,6 +248,26 @@ __device__ inline void store_state(PeakRec* r, const PeakState& s) {
     M::st(&r->status, s.status);
     M::st(&r->iter, s.iter);
     M::st(&r->added, s.added);
+  }
+}
+
+template <>
+__device__ inline void store_state<MemDev>(PeakRec* r, const PeakState& s) {
+  const int l = lane_id();
+  long long v = 0;
+#pragma unroll
+  for (int q = 0; q < kNFit; q++)
+    if (l == q) v = __double_as_longlong(s.prm[q]);
+  if (l == 7) v = __double_as_longlong(s.lam);
+  if (l == 8) v = __double_as_longlong(s.err);
+  if (l == 9) v = (long long)(((unsigned long long)(unsigned)s.yi << 32) | (unsigned)s.xi);
+  if (l == 10) v = (long long)(((unsigned long long)(unsigned)s.iter << 32) | (unsigned)s.status);
+  // word 11 holds `added` and the slot's owner frame, which the fit never changes
+  if (l < 10) {
+    MemDev::st(reinterpret_cast<long long*>(r) + l, v);
+  } else if (l == 10) {
+    MemDev::st(reinterpret_cast<long long*>(r) + 10, v);
+    MemDev::st(&r->added, s.added);
   }
 }
 
