@@ -75,14 +75,20 @@ def parse():
     p.add_argument("--margin", type=int, default=2)
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
-    p.add_argument("--serve-loads", default="0.05,0.2",
+    p.add_argument("--serve-loads", default="0.05,0.1,0.2",
                    help="offered loads of the serving leg, as fractions of 16 cameras at U(50, 400) frames/s")
     p.add_argument("--latency-frames", type=int, default=30,
                    help="frames fitted one at a time for the single-frame latency leg (0: skip)")
     p.add_argument("--team", type=int, default=64,
                    help="peaks per workgroup of the frame-per-team fitter in the latency and serving legs")
-    p.add_argument("--serve-slots", type=int, default=4,
-                   help="frame slots per serving worker in the latency set-up of the serving leg")
+    p.add_argument("--serve-workers", type=int, default=2,
+                   help="workers of the serving leg's latency set-up")
+    p.add_argument("--serve-slots", type=int, default=32,
+                   help="frame slots per worker in the serving leg's latency set-up")
+    p.add_argument("--serve-team", type=int, default=250,
+                   help="peaks per workgroup in the serving leg's latency set-up")
+    p.add_argument("--no-dense", action="store_true",
+                   help="skip the high-density leg (one 2048^2 frame, 20000 emitters, 2d model)")
     p.add_argument("--no-uneven", action="store_true",
                    help="skip the batched-uneven-frames leg (sCMOS 3D, 128^2..2048^2, autotuned)")
     p.add_argument("--precision", choices=("both", "mixed", "fp64"), default="both",
@@ -356,24 +362,76 @@ def latency_leg(args, images, backgrounds, rows, counts_np, log):
     return out
 
 
+def dense_leg(log, seed=20261210):
+    """
+    High-density overlap stress: one 2048^2 frame with 20000 emitters
+    (~4.6 /um^2 at 160 nm pixels), '2d' model (free width, sigma U(1.2, 2.0)),
+    heights U(200, 5000), background 100, Poisson noise.  FP64 and mixed,
+    one workgroup for the frame and the team fitter (500 peaks per
+    workgroup); time = doFit; the team result must equal the one-workgroup
+    result.
+    """
+    import numpy as np
+    from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
+    from storm_analysis_amd.diagnostics import synth
+    rng = np.random.default_rng(seed)
+    fr = synth.frame_2d_dense(rng, size=2048, n=20000)
+    kw = {"roi_size": ROI, "sigma_range": [0.96, 8.0]}
+    out = {"frame_size": 2048, "emitters": int(len(fr.peaks["x"])), "model": "2d", "runs": []}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        for team in (0, 500):
+            bf = dao_batch.BatchFitter([fr.image.shape], [len(fr.peaks["x"])], model="2d", precision=prec, **kw)
+            bf.setTuning(0, 2, team)
+            bf.setTiming(True)
+            bf.newImages([fr.image])
+            bf.newBackgrounds([np.full(fr.image.shape, 100.0)])
+            bf.newPeaks([fr.peaks], "hdf5")
+            t0 = time.perf_counter()
+            bf.doFit()
+            dt = time.perf_counter() - t0
+            p = bf.getPeaks(0)
+            c = bf.getCounters(0)
+            sweeps = bf.stats()["sweeps"]
+            bf.cleanup()
+            res[(prec, team)] = p
+            run = {"precision": prec, "team": team, "fit_s": dt, "converged": int((p["status"] == D.CONVERGED).sum()),
+                   "running_at_end": int((p["status"] == D.RUNNING).sum()), "lm_cycles": int(c["n_iterations"]),
+                   "sweeps": int(sweeps)}
+            out["runs"].append(run)
+            log("dense 2048^2 %s team %d: %.2f s, %d converged, %d LM cycles, %d sweeps" %
+                (prec, team, dt, run["converged"], run["lm_cycles"], run["sweeps"]))
+    for prec in ("mixed", "fp64"):
+        a, b = res[(prec, 0)], res[(prec, 500)]
+        out["%s_team_vs_one" % prec] = {"status_equal": bool(np.array_equal(a["status"], b["status"])),
+                                        "max_dxy_px": float(np.hypot(a["x"] - b["x"], a["y"] - b["y"]).max())}
+    a, b = res[("mixed", 0)], res[("fp64", 0)]
+    d = np.hypot(a["x"] - b["x"], a["y"] - b["y"])
+    ok = (a["status"] == D.CONVERGED) & (b["status"] == D.CONVERGED)
+    out["mixed_vs_fp64"] = {"flag_agreement": float((a["status"] == b["status"]).mean()),
+                            "frac_dxy_within_1nm": float((d[ok] < TOL_MIXED_XY_PX).mean()),
+                            "dxy_px_p50": float(np.percentile(d[ok], 50)), "dxy_px_p99": float(np.percentile(d[ok], 99))}
+    return out
+
+
 def serve_leg(args, precision, log):
     """
-    Continuous-batching server (8 workers, slots of 1024^2 / 4000 peaks)
-    under 16 simulated cameras with Poisson arrivals, sensors of 256^2,
-    512^2 or 1024^2 and ~1000 emitters per 512^2, Z model; p50/p99 latency
-    and fits/s per offered load.  Two set-ups: throughput (32 slots per
-    worker, one workgroup per frame) and latency (args.serve_slots slots,
-    frame-per-team fitter with one workgroup per args.team peaks).
+    Continuous-batching server (slots of 1024^2 / 4000 peaks) under 16
+    simulated cameras with Poisson arrivals, sensors of 256^2, 512^2 or
+    1024^2 and ~1000 emitters per 512^2, Z model; p50/p99 latency and fits/s
+    per offered load.  Two set-ups: throughput (8 workers x 32 slots, one
+    workgroup per frame) and latency (args.serve_workers workers x
+    args.serve_slots slots, team fitter with one workgroup per
+    args.serve_team peaks: a 1024^2 frame gets 16 CUs, and few workers keep
+    their launches on separate hardware queues).
     """
     from storm_analysis_amd.sa_library import dao_serve
     from storm_analysis_amd.diagnostics import serve_sim
     pools = serve_sim.make_pools((256, 512, 1024))
-    setups = {"throughput": dict(slots=32, team=0, margin=2), "latency": dict(slots=args.serve_slots, team=args.team,
-                                                                             margin=1)}
-    base = dict(workers=8, max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI, precision=precision,
-                **zkw())
-    out = {"server": {"workers": 8, "max_shape": [1024, 1024], "max_peaks": 4000, "precision": precision,
-                      "setups": setups},
+    setups = {"throughput": dict(workers=8, slots=32, team=0, margin=2),
+              "latency": dict(workers=args.serve_workers, slots=args.serve_slots, team=args.serve_team, margin=1)}
+    base = dict(max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI, precision=precision, **zkw())
+    out = {"server": {"max_shape": [1024, 1024], "max_peaks": 4000, "precision": precision, "setups": setups},
            "cameras": 16, "rates_fps": [50, 400], "sizes": [256, 512, 1024], "runs": []}
     for name, su in setups.items():
         srv_kw = dict(base, **su)
@@ -475,6 +533,10 @@ def main():
     if not args.no_uneven:
         uneven = uneven_leg(log)
 
+    dense = None
+    if not args.no_dense:
+        dense = dense_leg(log)
+
     serving = None
     if args.serve_seconds > 0:
         serving = serve_leg(args, head, log)
@@ -509,6 +571,7 @@ def main():
         "accuracy": accuracy,
         "mixed_xy_bar_px": TOL_MIXED_XY_PX,
         "uneven_batch": uneven,
+        "dense_2048": dense,
         "serving": serving,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
