@@ -155,6 +155,33 @@ constexpr int kTeamDone = -1;
 constexpr int kTeamSeats = 8;  // seats (one per wave) reserved per workgroup
 constexpr int kMaxTeam = 64;   // workgroups per frame
 
+// ROI pixels per lane of the kernel instance that fits an ROI of nroi pixels.
+__host__ __device__ inline int ppl_for(int nroi) {
+  int p = (nroi + kWave - 1) / kWave;
+  if (p <= 2) return 2;
+  if (p <= 4) return 4;
+  if (p <= 6) return 6;
+  if (p <= 8) return 8;
+  if (p <= 12) return 12;
+  return 16;
+}
+
+// An LM turn that may still be undone journals, for each ROI position it
+// loads (the start and every new candidate position), the pixels' fit state
+// as it was before the turn touched them: a snapshot of (3 PPL + 1) * 64
+// words -- words s * 64 + l, (PPL + s) * 64 + l and (2 PPL + s) * 64 + l the
+// foreground, background sum and cached transformed fit of slot s of lane l,
+// word 3 PPL * 64 + l the lane's stale bits (low 32 bits) and the ROI origin
+// (xi | yi << 16, high 32 bits).  Restoring the exact values (not adding the
+// stamps back) matters: a rejected candidate's stamp can be so large that
+// adding and removing it wipes out the pixel's other contributions, and an
+// undo must not do that a second time.  Covering counts are integers and are
+// undone by arithmetic.  A turn that would load more positions than
+// kJournalSnaps gives up like one that leaves its box.
+constexpr int kJournalSnaps = 4;
+__host__ __device__ constexpr long long journal_snap_words(int ppl) { return (long long)(3 * ppl + 1) * kWave; }
+__host__ __device__ constexpr long long journal_words(int ppl) { return kJournalSnaps * journal_snap_words(ppl); }
+
 struct FitArgs {
   Model m;
   Pix px;
@@ -165,7 +192,8 @@ struct FitArgs {
   int* run_list;        // per-slot scratch: RUNNING peaks of the sweep (LM runs)
   int4* box;            // per-slot scratch: op boxes
   int* op_delta;        // per-slot scratch: counter increments of an op
-  int* op_done;         // per-slot scratch: op finished in the current phase
+  int* op_done;         // per-slot scratch: op finished in the current phase (LM: 1 + snapshots journaled)
+  double* journal;      // per-slot scratch: cache snapshots of an LM turn (journal_words(PPL) each)
   int* frame_counters;  // [F][C_COUNT]
   int* frame_sweeps;    // [F]: sweeps run by this launch
   const int* frame_list;

@@ -182,6 +182,7 @@ void Engine::alloc_sched(long long c) {
   d_override = dalloc<int4>(c);
   d_flag = dalloc<int>(c);
   d_run = dalloc<int>(c);
+  d_journal = dalloc<double>(c * journal_words(ppl_for(m.nroi)));
 }
 
 void Engine::free_sched() {
@@ -198,6 +199,7 @@ void Engine::free_sched() {
   dfree(d_override);
   dfree(d_flag);
   dfree(d_run);
+  dfree(d_journal);
 }
 
 // Grow every per-slot array to c slots, keeping the first `cap` records.
@@ -458,6 +460,7 @@ int Engine::fit_frames(int max_more) {
   a.roi_tab = d_roi;
   a.op_peak = sc.op_peak;
   a.run_list = d_run;
+  a.journal = d_journal;
   a.box = sc.box;
   a.op_delta = sc.op_delta;
   a.op_done = sc.op_flags;

@@ -171,6 +171,7 @@ class Engine {
   void run_ops(int kind, const double* rows, int fmt, int first_slot);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* d_run = nullptr;  // RUNNING list of the frame fitters (per slot)
+  double* d_journal = nullptr;  // LM turn journals of the frame fitters (per slot)
   int* d_sweeps = nullptr;
   int* d_flist = nullptr;
   int* d_err = nullptr;

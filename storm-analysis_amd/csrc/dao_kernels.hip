@@ -176,14 +176,16 @@ __global__ __launch_bounds__(256) void k_round(OpArgs a, int kind, const double*
   int r = 1;
   double* slice = lds[wv];
   switch (kind) {
-    case OP_LM:
-      r = lm_turn<PPL, SHAPE>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, a.sc.box[i], d);
+    case OP_LM: {
+      int nsnap = 0;
+      r = lm_turn<PPL, SHAPE>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, a.sc.box[i], d, nullptr, nsnap);
       if (r == 2 && lane_id() == 0) {
         // re-run in the next attempt with the whole frame as its box
         a.sc.esc_box[i] = make_int4(0, 0, fr.W, fr.H);
         a.sc.frame_bad[fi] = 1;
       }
       break;
+    }
     case OP_RECENTER:
       recenter_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       break;
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(256) void k_rollback(OpArgs a) {
   const int p = a.sc.op_peak[i];
   const int fi = a.pk.rec[p].frame;
   if (!a.sc.frame_bad[fi] || (a.sc.op_flags[i] & 1) == 0) return;
-  undo_turn<PPL>(a.m, a.px, a.pk, a.frames[fi], a.roi_tab, lds[wv], p);
+  undo_turn<PPL>(a.m, a.px, a.pk, a.frames[fi], a.roi_tab, lds[wv], p, nullptr, 0);
 }
 
 // ---------------------------------------------------------------------------

@@ -316,6 +316,57 @@ __device__ inline void lane_shape(const Model& m, const PeakState& s, double& t,
 }
 
 // ---------------------------------------------------------------------------
+// turn journal (layout at kJournalSnaps)
+// ---------------------------------------------------------------------------
+
+// Journals slot s of this lane (pixel record w).
+template <int PPL, class M>
+__device__ inline void journal_pixel(double* snap, int s, const PixRW& w) {
+  const int l = lane_id();
+  M::st(snap + s * kWave + l, w.f);
+  M::st(snap + (PPL + s) * kWave + l, w.bg);
+  M::st(snap + (2 * PPL + s) * kWave + l, w.tfi);
+}
+
+template <int PPL, class M>
+__device__ inline void journal_header(double* snap, unsigned stale, int xi, int yi) {
+  long long* h = reinterpret_cast<long long*>(snap) + 3 * PPL * kWave + lane_id();
+  M::st(h, (long long)stale | ((long long)(xi | (yi << 16)) << 32));
+}
+
+// Writes snapshot snap's foreground, background sums, cached values and
+// stale flags back to the pixels it was taken from (c supplies the ROI's
+// pixel offsets).
+template <int PPL, class M, class R>
+__device__ void journal_restore(const Pix& px, const Frame& fr, const R& c, const double* snap) {
+  const int l = lane_id();
+  const long long h = M::ld(reinterpret_cast<const long long*>(snap) + 3 * PPL * kWave + l);
+  const unsigned stale = (unsigned)h;
+  const int org = (int)(h >> 32);
+  const long long base = fr.pix0 + (long long)(org >> 16) * fr.W + (org & 0xffff);
+#pragma unroll
+  for (int s = 0; s < PPL; s++) {
+    if (c.ok(s)) {
+      PixRW* q = px.rw + base + c.koff[s];
+      M::st(&q->f, M::ld(snap + s * kWave + l));
+      M::st(&q->bg, M::ld(snap + (PPL + s) * kWave + l));
+      M::st(&q->tfi, M::ld(snap + (2 * PPL + s) * kWave + l));
+      M::st(&q->stale, (int)((stale >> s) & 1u));
+    }
+  }
+}
+
+// Puts back, newest first, the nsnap snapshots of journal jr: every pixel
+// ends with the state its first snapshot holds (counts aside).
+template <int PPL, class M, class R>
+__device__ void journal_unwind(const Pix& px, const Frame& fr, const R& c, const double* jr, int nsnap) {
+  for (int v = nsnap - 1; v >= 0; v--) {
+    dev_order();  // the previous writes of these pixels have completed
+    journal_restore<PPL, M>(px, fr, c, jr + v * journal_snap_words(PPL));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // ROI cache of one wave
 // ---------------------------------------------------------------------------
 
@@ -369,8 +420,10 @@ struct Roi {
     valid = false;
   }
 
+  // Loads the ROI at (xi, yi); with snap, also journals the pixels' fit
+  // state there (see kJournalSnaps).
   template <class M = MemL1>
-  __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
+  __device__ void load(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
 #pragma unroll
@@ -387,8 +440,10 @@ struct Roi {
         x(s) = r.x;
         rqe(s) = r.rqe;
         scm(s) = r.scm;
+        if (snap) journal_pixel<PPL, M>(snap, s, w);
       }
     }
+    if (snap) journal_header<PPL, M>(snap, stale, xi, yi);
     at_x = xi;
     at_y = yi;
     valid = true;
@@ -412,10 +467,10 @@ struct Roi {
   }
 
   template <class M = MemL1>
-  __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi) {
+  __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     if (valid && at_x == xi && at_y == yi) return;
     store<M>(px);
-    load<M>(px, fr, xi, yi);
+    load<M>(px, fr, xi, yi, snap);
   }
 
   __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLds<PPL>::kTab); }
@@ -735,7 +790,7 @@ struct RoiM {
   }
 
   template <class M = MemL1>
-  __device__ void load(const Pix& px, const Frame& fr, int xi, int yi) {
+  __device__ void load(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
 #pragma unroll
@@ -752,8 +807,10 @@ struct RoiM {
         x(s) = r.x;
         rqe(s) = (float)r.rqe;
         scm(s) = (float)r.scm;
+        if (snap) journal_pixel<PPL, M>(snap, s, w);
       }
     }
+    if (snap) journal_header<PPL, M>(snap, stale, xi, yi);
     at_x = xi;
     at_y = yi;
     valid = true;
@@ -777,10 +834,10 @@ struct RoiM {
   }
 
   template <class M = MemL1>
-  __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi) {
+  __device__ void move_to(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     if (valid && at_x == xi && at_y == yi) return;
     store<M>(px);
-    load<M>(px, fr, xi, yi);
+    load<M>(px, fr, xi, yi, snap);
   }
 
   __device__ void table(const Model& m, const PeakState& st) {
@@ -1095,15 +1152,19 @@ __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
 
 // One Levenberg-Marquardt turn of peak p: the body of the loop at
 // multi_fit.c:1089-1345.  The turn may touch only pixels inside `box`.
-// Returns 1 when the turn completed, 2 when a candidate ROI left the box: the
-// frame is then exactly as before the turn (the starting contribution is put
-// back) and the peak record is untouched.  d[] receives the turn's counter
-// increments.
+// Returns 1 when the turn completed, 2 when a candidate ROI left the box (or
+// the journal is full): the frame is then as before the turn (restored from
+// the journal) and the peak record is untouched.  d[] receives the turn's
+// counter increments.  With jr, the turn journals every ROI position it
+// loads into jr (nsnap snapshots, see kJournalSnaps), so that it can be
+// undone exactly; a turn that cannot be undone or leave its box (its box is
+// the whole frame) needs none.
 template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1>
 __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                       double* slice, int p, int4 box, int* d) {
+                       double* slice, int p, int4 box, int* d, double* jr, int& nsnap) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
   PeakRec* rec = pk.rec + p;
+  nsnap = 0;
   // The record keeps the turn's starting state until the turn ends, so
   // resets read it back from there instead of holding a second copy.
   PeakState w;
@@ -1116,7 +1177,8 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
 
   R c;
   c.init(m, roi_tab, fr, slice);
-  c.template load<M>(px, fr, w.xi, w.yi);
+  c.template load<M>(px, fr, w.xi, w.yi, jr);
+  if (jr) nsnap = 1;
   c.shape(m, w);
 
   double wJ[N];
@@ -1162,8 +1224,17 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
       escaped = true;
       break;
     }
+    double* snap = nullptr;
+    if (jr && (w.xi != c.at_x || w.yi != c.at_y)) {
+      if (nsnap == kJournalSnaps) {
+        escaped = true;
+        break;
+      }
+      snap = jr + nsnap * journal_snap_words(PPL);
+      nsnap++;
+    }
 
-    c.template move_to<M>(px, fr, w.xi, w.yi);
+    c.template move_to<M>(px, fr, w.xi, w.yi, snap);
     c.shape(m, w);
     c.stamp(w, +1);
     w.added++;
@@ -1197,15 +1268,19 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   }
 
   if (escaped) {
-    // Put the frame back: re-add the turn's starting contribution.  The
-    // escape is detected before the candidate touches any pixel.
+    // Put the frame back: re-add the turn's starting contribution (the
+    // escape is detected before the candidate touches any pixel), then
+    // restore the exact state of every pixel the turn touched from its
+    // journal.
     load_state<M>(rec, w);
     c.template move_to<M>(px, fr, w.xi, w.yi);
     c.shape(m, w);
     c.stamp(w, +1);
     c.template store<M>(px);
+    if (jr) journal_unwind<PPL, M>(px, fr, c, jr, nsnap);
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
+    nsnap = 0;
     return 2;
   }
 
@@ -1249,10 +1324,12 @@ __device__ void recenter_turn(const Model& m, const Pix& px, const Peaks& pk, co
 }
 
 // Add (sign +1) or remove (-1) `added` copies of state s's contribution with
-// atomic adds (undos; the callers order overlapping ones).
+// atomic adds (undos; the callers order overlapping ones).  With counts_only
+// just the covering counts change (the caller restores the rest from the
+// turn's journal).
 template <int PPL, class M = MemL1>
 __device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const Roi<PPL>& c,
-                           const PeakState& s, int sign) {
+                           const PeakState& s, int sign, bool counts_only) {
   const int n = s.added * sign;
   if (n == 0 || !origin_ok(m, fr, s.xi, s.yi)) return;
   double t, e;
@@ -1264,29 +1341,36 @@ __device__ void undo_stamp(const Model& m, const Pix& px, const Frame& fr, const
     double ey = __shfl(e, 32 + c.ry(q), kWave);
     if (c.ok(q)) {
       const long long k = base + c.koff[q];
-      const PixRO r = px.ro[k];
-      atomicAdd(&px.rw[k].f, (double)n * (s.prm[P_HEIGHT] * (ey * ex) * r.rqe));
-      atomicAdd(&px.rw[k].bg, (double)n * (s.prm[P_BG] * r.rqe + r.scm));
       atomicAdd(&px.rw[k].cnt, n);
-      M::st(&px.rw[k].stale, 1);
+      if (!counts_only) {
+        const PixRO r = px.ro[k];
+        atomicAdd(&px.rw[k].f, (double)n * (s.prm[P_HEIGHT] * (ey * ex) * r.rqe));
+        atomicAdd(&px.rw[k].bg, (double)n * (s.prm[P_BG] * r.rqe + r.scm));
+        M::st(&px.rw[k].stale, 1);
+      }
     }
   }
 }
 
 // Undo a completed LM turn of peak p: the image loses the peak's current
 // contribution and regains the one it had at the sweep's start (backup
-// record), and the peak returns to the backup.
+// record), and the peak returns to the backup.  With the turn's journal (jr,
+// nsnap snapshots) the pixels the turn touched return exactly to their state
+// before it (the counts by arithmetic); without, the contributions are
+// subtracted and added back and the touched pixels marked stale.
 template <int PPL, class M = MemL1>
 __device__ void undo_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                          double* slice, int p) {
+                          double* slice, int p, const double* jr, int nsnap) {
   PeakState w, s0;
   load_state<M>(pk.rec + p, w);
   load_state<M>(pk.bk + p, s0);
   Roi<PPL> c;
   c.init(m, roi_tab, fr, slice);
-  undo_stamp<PPL, M>(m, px, fr, c, w, -1);
-  undo_stamp<PPL, M>(m, px, fr, c, s0, +1);
+  const bool journaled = jr != nullptr && nsnap > 0;
+  undo_stamp<PPL, M>(m, px, fr, c, w, -1, journaled);
+  undo_stamp<PPL, M>(m, px, fr, c, s0, +1, journaled);
   store_state<M>(pk.rec + p, s0);
+  if (journaled) journal_unwind<PPL, M>(px, fr, c, jr, nsnap);
 }
 
 // daoNewPeaks body (dao_fit.c:1605-1751) for one new peak from a row of 4
@@ -1492,6 +1576,21 @@ __device__ int fpw_list(const FitArgs& a, const Frame& fr, int mode, int* wsums,
   return count;
 }
 
+// Journal of op slot `slot`.
+template <int PPL>
+__device__ inline double* op_journal(const FitArgs& a, int slot) {
+  return a.journal + (long long)slot * journal_words(PPL);
+}
+
+// Journal for an LM op with box b, or none: an op whose box is the whole
+// frame starts when every earlier op has finished and cannot leave its box,
+// so it is never undone.
+template <int PPL>
+__device__ inline double* op_journal(const FitArgs& a, const Frame& fr, int slot, int4 b) {
+  const bool whole = b.x == 0 && b.y == 0 && b.z == fr.W && b.w == fr.H;
+  return (a.journal == nullptr || whole) ? nullptr : op_journal<PPL>(a, slot);
+}
+
 // Wait until op j may start.  Returns false when an earlier op escaped (op j
 // must not run now) or the wait bound was hit.
 template <int NW>
@@ -1565,9 +1664,10 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
       int d[C_COUNT];
 #pragma unroll
       for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-      int r = 1;
+      int r = 1, nsnap = 0;
       if (kind == OP_LM) {
-        r = lm_turn<PPL, SHAPE, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d);
+        r = lm_turn<PPL, SHAPE, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, op_journal<PPL>(a, fr, slot, b),
+                                   nsnap);
       } else {
         recenter_turn<PPL, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       }
@@ -1582,7 +1682,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
           a.op_delta[(long long)slot * C_COUNT + q] = d[q];
           if (d[q]) atomicAdd(&sh.cnt[q], d[q]);
         }
-        a.op_done[slot] = 1;
+        a.op_done[slot] = 1 + nsnap;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       fpw_set_cur(&sh.cur[wv], kIdle);
@@ -1593,8 +1693,9 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
     if (wv == 0) {
       for (int k = n - 1; k > e; k--) {
         const int slot = fr.peak0 + k;
-        if (a.op_done[slot]) {
-          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, list[slot]);
+        const int done = a.op_done[slot];
+        if (done) {
+          undo_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, list[slot], op_journal<PPL>(a, slot), done - 1);
           if (lane == 0) {
             for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= a.op_delta[(long long)slot * C_COUNT + q];
             a.op_done[slot] = 0;
@@ -1837,9 +1938,10 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
     int d[C_COUNT];
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
-    int r = 1;
+    int r = 1, nsnap = 0;
     if (kind == OP_LM) {
-      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d);
+      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d,
+                                         op_journal<PPL>(a, fr, slot, b), nsnap);
     } else {
       recenter_turn<PPL, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
     }
@@ -1856,7 +1958,7 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
     }
     dev_order();  // the turn's stores have completed
     if (lane == 0) {
-      MemDev::st(&a.op_done[slot], 1);
+      MemDev::st(&a.op_done[slot], 1 + nsnap);
       MemDev::add(&ctl->done, 1);
     }
   }
@@ -1934,8 +2036,10 @@ __device__ __noinline__ void team_transition(const FitArgs& a, const Frame& fr, 
     if (wv == 0) {
       for (int k = n - 1; k > esc; k--) {
         const int slot = fr.peak0 + k;
-        if (uniform_i(MemDev::ld(&a.op_done[slot]))) {
-          undo_turn<PPL, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, uniform_i(MemDev::ld(&a.run_list[slot])));
+        const int done = uniform_i(MemDev::ld(&a.op_done[slot]));
+        if (done) {
+          undo_turn<PPL, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, uniform_i(MemDev::ld(&a.run_list[slot])),
+                                 op_journal<PPL>(a, slot), done - 1);
           dev_order();
           if (lane == 0) {
             for (int q = 0; q < C_COUNT; q++) sh.cnt[q] -= MemDev::ld(&a.op_delta[(long long)slot * C_COUNT + q]);
@@ -2045,15 +2149,6 @@ __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
 // launch helpers
 // ---------------------------------------------------------------------------
 
-static int ppl_for(int nroi) {
-  int p = (nroi + kWave - 1) / kWave;
-  if (p <= 2) return 2;
-  if (p <= 4) return 4;
-  if (p <= 6) return 6;
-  if (p <= 8) return 8;
-  if (p <= 12) return 12;
-  return 16;
-}
 
 #define DAO_PPL_SWITCH(nroi, KCALL)                    \
   switch (ppl_for(nroi)) {                             \

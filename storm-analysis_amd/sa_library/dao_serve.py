@@ -32,21 +32,37 @@ from . import dao_batch
 from .dao_fit_c import MultiFitterException
 
 # device bytes per slot pixel (two 32-byte pixel records + the Anscombe
-# image) and per slot peak (two 96-byte peak records + scheduling scratch)
+# image) and per slot peak (two 96-byte peak records + scheduling scratch),
+# besides the peak's LM turn journal (journal_bytes)
 BYTES_PER_PIXEL = 72
 BYTES_PER_PEAK = 840
+JOURNAL_SNAPSHOTS = 4
 
 
-def slot_bytes(max_shape, max_peaks):
+def roi_pixels(roi_size):
+    """Pixels of the round ROI of a roi_size fitting area (the fitter's ROI table)."""
+    c = 0.5 * roi_size
+    d = numpy.arange(roi_size) - c + 0.5
+    return int(((d[:, None] ** 2 + d[None, :] ** 2) <= c * c).sum())
+
+
+def journal_bytes(roi_size):
+    """Device bytes of one peak's LM turn journal (JOURNAL_SNAPSHOTS ROI snapshots)."""
+    per_lane = -(-roi_pixels(roi_size) // 64)
+    ppl = next(p for p in (2, 4, 6, 8, 12, 16) if per_lane <= p or p == 16)
+    return JOURNAL_SNAPSHOTS * (3 * ppl + 1) * 64 * 8
+
+
+def slot_bytes(max_shape, max_peaks, roi_size=18):
     """Device bytes one slot of max_shape pixels and max_peaks peaks holds."""
-    return max_shape[0] * max_shape[1] * BYTES_PER_PIXEL + max_peaks * BYTES_PER_PEAK
+    return max_shape[0] * max_shape[1] * BYTES_PER_PIXEL + max_peaks * (BYTES_PER_PEAK + journal_bytes(roi_size))
 
 
-def pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb=None):
+def pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb=None, roi_size=18):
     """(workers, slots) after admission by the HBM budget (fewer slots first)."""
     if hbm_budget_gb is None:
         return workers, slots
-    fit = int(hbm_budget_gb * 1e9 // slot_bytes(max_shape, max_peaks))
+    fit = int(hbm_budget_gb * 1e9 // slot_bytes(max_shape, max_peaks, roi_size))
     if fit < 1:
         raise MultiFitterException("HBM budget holds no slot of %s" % (tuple(max_shape),))
     while workers * slots > fit and slots > 1:
@@ -96,10 +112,11 @@ class FitServer(object):
         self.max_shape = tuple(max_shape)
         self.max_peaks = max_peaks
         self.max_iterations = max_iterations
-        workers, slots = pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb)
+        roi_size = fitter_kw.get("roi_size", 18)
+        workers, slots = pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb, roi_size)
         self.workers = workers
         self.slots = slots
-        self.slot_bytes = slot_bytes(max_shape, max_peaks)
+        self.slot_bytes = slot_bytes(max_shape, max_peaks, roi_size)
         self.fitters = [dao_batch.BatchFitter([self.max_shape] * slots, [max_peaks] * slots, **fitter_kw)
                         for _ in range(workers)]
         for bf in self.fitters:

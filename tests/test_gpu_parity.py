@@ -53,8 +53,14 @@ def assert_parity(g, r, tol_pos=TOL_POS, tol_rel=TOL_REL, counters=True):
         assert g["iterations_total"] == r["iterations_total"]
         for c in COUNTERS:
             assert g[c] == r[c], (c, g[c], r[c])
-    scale = max(1.0, numpy.abs(r["fit_image"]).max())
-    assert numpy.abs(g["fit_image"] - r["fit_image"]).max() < 1e-6 * scale
+    # A frame so dense that rejected candidates overflow the fit image holds
+    # inf / nan pixels: the same ones, with the same values, on both sides.
+    gi, ri = g["fit_image"], r["fit_image"]
+    fin = numpy.isfinite(ri)
+    numpy.testing.assert_array_equal(gi[~fin], ri[~fin])
+    if fin.any():
+        scale = max(1.0, numpy.abs(ri[fin]).max())
+        assert numpy.abs(gi[fin] - ri[fin]).max() < 1e-6 * scale
 
 
 def both(*args, **kwds):

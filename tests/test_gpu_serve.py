@@ -108,14 +108,20 @@ def test_server_admission_limits():
 
 
 def test_hbm_budget_sizes_the_pool():
+    # an 18-pixel fitting area holds a round ROI of 256 pixels: 4 per lane,
+    # so a journal snapshot is (3 * 4 + 1) * 64 words
+    assert dao_serve.roi_pixels(18) == 256
+    assert dao_serve.journal_bytes(18) == dao_serve.JOURNAL_SNAPSHOTS * 13 * 64 * 8 == 26624
     per_slot = dao_serve.slot_bytes((1024, 1024), 4000)
-    assert per_slot == 1024 * 1024 * dao_serve.BYTES_PER_PIXEL + 4000 * dao_serve.BYTES_PER_PEAK
+    assert per_slot == 1024 * 1024 * dao_serve.BYTES_PER_PIXEL + 4000 * (dao_serve.BYTES_PER_PEAK + 26624)
     # no budget: as asked
     assert dao_serve.pool_size(4, 32, (1024, 1024), 4000) == (4, 32)
-    # 2 GB holds 25 such slots: fewer slots per worker first, then fewer workers
+    # 2 GB holds 10 such slots: fewer slots per worker first, then fewer workers
     w, s = dao_serve.pool_size(4, 32, (1024, 1024), 4000, hbm_budget_gb=2.0)
-    assert w * s <= int(2e9 // per_slot) and (w, s) == (4, 6)
+    assert w * s <= int(2e9 // per_slot) and (w, s) == (4, 2)
     w, s = dao_serve.pool_size(4, 32, (1024, 1024), 4000, hbm_budget_gb=0.2)
-    assert (w, s) == (2, 1)
+    assert (w, s) == (1, 1)
+    # smaller ROIs need smaller journals
+    assert dao_serve.slot_bytes((1024, 1024), 4000, roi_size=10) < per_slot
     with pytest.raises(D.MultiFitterException):
         dao_serve.pool_size(4, 32, (1024, 1024), 4000, hbm_budget_gb=0.01)

@@ -13,7 +13,7 @@ mixed fitter), and for a batch of uneven frames.
 import numpy
 import pytest
 
-from test_gpu_parity import assert_parity, fit_frame
+from test_gpu_parity import TOL_POS, assert_parity, fit_frame
 from test_gpu_batch import assert_frame_parity, oracle_fit, scmos_frames
 from storm_analysis_amd.sa_library import dao_batch
 from storm_analysis_amd.sa_library import dao_fit_c as D
@@ -121,3 +121,58 @@ def test_team_other_models_match_oracle(cls, kw):
     g = fit_team(10, cls, kw, fr.image, fr.peaks)
     r = fit_frame("oracle", cls, kw, fr.image, fr.peaks)
     assert_parity(g, r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size,density,team", [(128, 1.2, 0), (128, 1.2, 50), (128, 2.0, 0), (96, 4.6, 0),
+                                               (96, 4.6, 100)])
+def test_high_density_overlap_matches_oracle(size, density, team):
+    """
+    The overlap stress of the high-density configuration at sizes the oracle
+    runs in seconds: 2D model, free width, sigma U(1.2, 2.0), up to 4.6
+    emitters/um^2 at 160 nm pixels (a start peak for every emitter; at 4.6
+    nearly all peaks end as ERROR, which exercises the lost / negative-height
+    paths in order), one workgroup and the team fitter.
+    """
+    n = int(round(size * size * 0.0256 * density))
+    rng = numpy.random.default_rng(synth.SEED_BASE + (50 if density > 4 else 51))
+    fr = synth.frame_2d_dense(rng, size=size, n=n)
+    kw = {"roi_size": 18, "sigma_range": [0.96, 8.0]}
+    g = fit_team(team, D.MultiFitter2D, kw, fr.image, fr.peaks)
+    r = fit_frame("oracle", D.MultiFitter2D, kw, fr.image, fr.peaks)
+    assert_parity(g, r)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("waves,margin", [(1, 20), (8, 2), (8, 20), (2, 2)])
+def test_high_density_rollbacks_match_oracle(waves, margin):
+    """
+    Rollbacks at 2.0 emitters/um^2: with a small margin turns leave their
+    boxes after moving, with a large one they fill their journals (more ROI
+    positions than snapshots); either way the turn and every later turn that
+    ran beside it are undone and run again.  Rejected candidates here stamp
+    contributions large enough to wipe out a pixel's other contributions when
+    removed again, so only an exact restore of the pixels keeps the
+    reference's results -- whatever the waves per workgroup and the margin.
+    """
+    size, density = 128, 2.0
+    n = int(round(size * size * 0.0256 * density))
+    rng = numpy.random.default_rng(synth.SEED_BASE + 51)
+    fr = synth.frame_2d_dense(rng, size=size, n=n)
+    kw = {"roi_size": 18, "sigma_range": [0.96, 8.0]}
+    r = fit_frame("oracle", D.MultiFitter2D, kw, fr.image, fr.peaks)
+    bf = dao_batch.BatchFitter([fr.image.shape], [n], model="2d", **kw)
+    try:
+        bf.setTuning(waves, margin, 0)
+        bf.newImages([fr.image])
+        bf.newBackgrounds([numpy.full(fr.image.shape, 100.0)])
+        bf.newPeaks([fr.peaks], "hdf5")
+        bf.doFit()
+        g = bf.getPeaks(0)
+    finally:
+        bf.cleanup()
+    numpy.testing.assert_array_equal(g["status"], r["status"])
+    numpy.testing.assert_array_equal(g["iterations"], r["iterations"])
+    ok = r["status"] != D.ERROR
+    for k in ("x", "y"):
+        assert numpy.abs(g[k] - r[k])[ok].max() < TOL_POS, k
