@@ -115,9 +115,9 @@ void daoBatchSetTiming(daoBatch *b, int on);
  * CU), the reserved-box margin in pixels (>= 0; -1 keeps it) and the team
  * size: a frame with n peaks is fitted by min(64, ceil(n / team))
  * workgroups on as many CUs (0: one workgroup per frame; -1 keeps it), which
- * shortens the frame's latency.  Results are the same for every setting
- * (bit-identical unless a reserved box overflows and the rollback's
- * arithmetic undo runs). */
+ * shortens the frame's latency.  Results are the same for every setting: a
+ * turn that leaves its reserved box is rolled back exactly, from the pixel
+ * journal of every turn undone. */
 void daoBatchSetTuning(daoBatch *b, int waves, int margin, int team);
 void daoBatchStats(daoBatch *b, double *out6);
 

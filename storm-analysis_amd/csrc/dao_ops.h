@@ -115,6 +115,9 @@ struct MemL1 {
   __device__ static T ld(const T* p) { return *p; }
   template <class T>
   __device__ static void st(T* p, const T& v) { *p = v; }
+  // stores of data that is rarely read back (turn journals): streamed
+  template <class T>
+  __device__ static void st_stream(T* p, const T& v) { __builtin_nontemporal_store(v, p); }
 };
 
 struct MemDev {
@@ -128,6 +131,8 @@ struct MemDev {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar access");
     __hip_atomic_store(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  template <class T>
+  __device__ static void st_stream(T* p, const T& v) { st(p, v); }
   // atomic read-modify-writes of the team's control words
   __device__ static int add(int* p, int v) {
     return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -319,19 +324,19 @@ __device__ inline void lane_shape(const Model& m, const PeakState& s, double& t,
 // turn journal (layout at kJournalSnaps)
 // ---------------------------------------------------------------------------
 
-// Journals slot s of this lane (pixel record w).
+// Journals slot s of this lane.
 template <int PPL, class M>
-__device__ inline void journal_pixel(double* snap, int s, const PixRW& w) {
+__device__ inline void journal_pixel(double* snap, int s, double f, double bg, double tfi) {
   const int l = lane_id();
-  M::st(snap + s * kWave + l, w.f);
-  M::st(snap + (PPL + s) * kWave + l, w.bg);
-  M::st(snap + (2 * PPL + s) * kWave + l, w.tfi);
+  M::st_stream(snap + s * kWave + l, f);
+  M::st_stream(snap + (PPL + s) * kWave + l, bg);
+  M::st_stream(snap + (2 * PPL + s) * kWave + l, tfi);
 }
 
 template <int PPL, class M>
 __device__ inline void journal_header(double* snap, unsigned stale, int xi, int yi) {
   long long* h = reinterpret_cast<long long*>(snap) + 3 * PPL * kWave + lane_id();
-  M::st(h, (long long)stale | ((long long)(xi | (yi << 16)) << 32));
+  M::st_stream(h, (long long)stale | ((long long)(xi | (yi << 16)) << 32));
 }
 
 // Writes snapshot snap's foreground, background sums, cached values and
@@ -356,13 +361,15 @@ __device__ void journal_restore(const Pix& px, const Frame& fr, const R& c, cons
   }
 }
 
-// Puts back, newest first, the nsnap snapshots of journal jr: every pixel
-// ends with the state its first snapshot holds (counts aside).
+// Puts back, newest first, the nsnap snapshots of journal jr (snapshot 0 in
+// `stage` instead when given): every pixel ends with the state its first
+// snapshot holds (counts aside).
 template <int PPL, class M, class R>
-__device__ void journal_unwind(const Pix& px, const Frame& fr, const R& c, const double* jr, int nsnap) {
+__device__ void journal_unwind(const Pix& px, const Frame& fr, const R& c, const double* jr, int nsnap,
+                               const double* stage = nullptr) {
   for (int v = nsnap - 1; v >= 0; v--) {
     dev_order();  // the previous writes of these pixels have completed
-    journal_restore<PPL, M>(px, fr, c, jr + v * journal_snap_words(PPL));
+    journal_restore<PPL, M>(px, fr, c, (v == 0 && stage) ? stage : jr + v * journal_snap_words(PPL));
   }
 }
 
@@ -440,10 +447,16 @@ struct Roi {
         x(s) = r.x;
         rqe(s) = r.rqe;
         scm(s) = r.scm;
-        if (snap) journal_pixel<PPL, M>(snap, s, w);
       }
     }
-    if (snap) journal_header<PPL, M>(snap, stale, xi, yi);
+    // journal after all loads are issued: a store between them would hold
+    // back the later loads (the compiler cannot tell the arrays apart)
+    if (snap) {
+#pragma unroll
+      for (int s = 0; s < PPL; s++)
+        if (ok(s)) journal_pixel<PPL, M>(snap, s, f[s], bg[s], tfi[s]);
+      journal_header<PPL, M>(snap, stale, xi, yi);
+    }
     at_x = xi;
     at_y = yi;
     valid = true;
@@ -793,8 +806,10 @@ struct RoiM {
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
+    double tfi[PPL];  // the cached values in FP64, for the journal
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
+      tfi[s] = 0.0;
       if (ok(s)) {
         const PixRW w = M::ld(px.rw + k(s));
         const GlobalT<PixRO>* ro = gptr(px.ro + k(s));
@@ -802,15 +817,21 @@ struct RoiM {
         f[s] = w.f;
         bg[s] = w.bg;
         dfi[s] = (float)(w.tfi - r.x);
+        tfi[s] = w.tfi;
         cnt[s] = w.cnt;
         if (w.stale) stale |= 1u << s;
         x(s) = r.x;
         rqe(s) = (float)r.rqe;
         scm(s) = (float)r.scm;
-        if (snap) journal_pixel<PPL, M>(snap, s, w);
       }
     }
-    if (snap) journal_header<PPL, M>(snap, stale, xi, yi);
+    // journal after all loads are issued (see Roi::load)
+    if (snap) {
+#pragma unroll
+      for (int s = 0; s < PPL; s++)
+        if (ok(s)) journal_pixel<PPL, M>(snap, s, f[s], bg[s], tfi[s]);
+      journal_header<PPL, M>(snap, stale, xi, yi);
+    }
     at_x = xi;
     at_y = yi;
     valid = true;
@@ -1156,12 +1177,12 @@ __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
 // the journal is full): the frame is then as before the turn (restored from
 // the journal) and the peak record is untouched.  d[] receives the turn's
 // counter increments.  With jr, the turn journals every ROI position it
-// loads into jr (nsnap snapshots, see kJournalSnaps), so that it can be
-// undone exactly; a turn that cannot be undone or leave its box (its box is
-// the whole frame) needs none.
+// loads into jr (nsnap snapshots, see kJournalSnaps; the first one into
+// `stage` instead when given), so that it can be undone exactly; a turn that
+// cannot be undone or leave its box (its box is the whole frame) needs none.
 template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1>
 __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                       double* slice, int p, int4 box, int* d, double* jr, int& nsnap) {
+                       double* slice, int p, int4 box, int* d, double* jr, int& nsnap, double* stage = nullptr) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
   PeakRec* rec = pk.rec + p;
   nsnap = 0;
@@ -1177,7 +1198,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
 
   R c;
   c.init(m, roi_tab, fr, slice);
-  c.template load<M>(px, fr, w.xi, w.yi, jr);
+  c.template load<M>(px, fr, w.xi, w.yi, (jr && stage) ? stage : jr);
   if (jr) nsnap = 1;
   c.shape(m, w);
 
@@ -1277,7 +1298,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
     c.shape(m, w);
     c.stamp(w, +1);
     c.template store<M>(px);
-    if (jr) journal_unwind<PPL, M>(px, fr, c, jr, nsnap);
+    if (jr) journal_unwind<PPL, M>(px, fr, c, jr, nsnap, stage);
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
     nsnap = 0;
@@ -1517,6 +1538,22 @@ constexpr int kNoEscape = 0x7fffffff;
 constexpr int kIdle = 0x7fffffff;  // wave holds no op
 constexpr int kPending = -1;       // wave is taking an op (index not known yet)
 
+// Launched with nw * 64 threads (nw <= NW) and nw * kSlice doubles of
+// dynamic LDS (fit_frames_lds): fewer waves per frame leave room for more
+// workgroups per CU.
+template <int PPL, bool MP>
+constexpr int fit_frames_slice() {
+  return MP ? RoiLdsM<PPL>::kDoubles : RoiLds<PPL>::kDoubles;
+}
+
+// k_fit_frames stages each turn's first journal snapshot in the wave's LDS
+// (after its slice) when that fits beside eight waves' slices: the snapshot
+// reaches HBM only if the turn can still be undone when it ends.
+template <int PPL>
+constexpr int fit_frames_stage() {
+  return PPL <= 4 ? (int)journal_snap_words(PPL) : 0;
+}
+
 template <int NW>
 struct FpwShared {
   int next;         // next op to hand out
@@ -1665,9 +1702,10 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
 #pragma unroll
       for (int q = 0; q < C_COUNT; q++) d[q] = 0;
       int r = 1, nsnap = 0;
+      double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
+      double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
       if (kind == OP_LM) {
-        r = lm_turn<PPL, SHAPE, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, op_journal<PPL>(a, fr, slot, b),
-                                   nsnap);
+        r = lm_turn<PPL, SHAPE, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
       } else {
         recenter_turn<PPL, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       }
@@ -1676,13 +1714,32 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
+      int done = 1 + nsnap;
+      if (jr && stage) {
+        // The staged snapshot goes to the op's journal if an earlier op is
+        // still running or has left its box (the op may then be undone);
+        // otherwise every earlier op has finished for good and this one
+        // stays.  An earlier op that escapes sets esc before it leaves its
+        // seat, so a seat read as free means esc is already visible.
+        bool open = false;
+        if (lane < (int)(blockDim.x >> 6) && lane != wv) {
+          const int c = __hip_atomic_load(&sh.cur[lane], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          open = c >= 0 && c < j;
+        }
+        open = __any(open) || __hip_atomic_load(&sh.esc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < j;
+        if (open) {
+          for (int q = lane; q < (int)journal_snap_words(PPL); q += kWave) MemL1::st_stream(jr + q, stage[q]);
+        } else {
+          done = 1;  // never undone: no journal
+        }
+      }
       if (lane == 0) {
 #pragma unroll
         for (int q = 0; q < C_COUNT; q++) {
           a.op_delta[(long long)slot * C_COUNT + q] = d[q];
           if (d[q]) atomicAdd(&sh.cnt[q], d[q]);
         }
-        a.op_done[slot] = 1 + nsnap;
+        a.op_done[slot] = done;
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       fpw_set_cur(&sh.cur[wv], kIdle);
@@ -1710,17 +1767,9 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
   }
 }
 
-// Launched with nw * 64 threads (nw <= NW) and nw * kSlice doubles of
-// dynamic LDS (fit_frames_lds): fewer waves per frame leave room for more
-// workgroups per CU.
-template <int PPL, bool MP>
-constexpr int fit_frames_slice() {
-  return MP ? RoiLdsM<PPL>::kDoubles : RoiLds<PPL>::kDoubles;
-}
-
 template <int PPL, int SHAPE, int NW, bool MP>
 __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
-  constexpr int kSlice = fit_frames_slice<PPL, MP>();
+  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
   __shared__ FpwShared<NW> sh;
   extern __shared__ double lds_dyn[];
   double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
@@ -2176,7 +2225,7 @@ static inline dim3 flat_grid(long long n) { return dim3((unsigned)((n + 255) / 2
 template <int PPL, int SHP, int NW, bool MP>
 void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
   const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
-  const size_t bytes = sizeof(double) * (size_t)nw * fit_frames_slice<PPL, MP>();
+  const size_t bytes = sizeof(double) * (size_t)nw * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;  // larger dynamic LDS must be enabled per kernel
   if (bytes > allowed) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP>),
