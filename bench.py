@@ -49,9 +49,9 @@ ALG_BYTES_PER_PIXEL = 36
 # all of them).
 MEASURED_TRAFFIC = {
     "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
-              "fetch_kib": 93042819.5, "write_kib": 85221069.0, "fetch_scale": 2.0, "write_scale": 1.0},
+              "fetch_kib": 95332820.6, "write_kib": 121726713.1, "fetch_scale": 2.0, "write_scale": 1.0},
     "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
-             "fetch_kib": 108260797.4, "write_kib": 101654105.6, "fetch_scale": 2.0, "write_scale": 1.0},
+             "fetch_kib": 116015976.5, "write_kib": 154402598.2, "fetch_scale": 2.0, "write_scale": 1.0},
 }
 # accuracy bar of mixed precision against FP64 for x and y
 TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
