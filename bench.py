@@ -73,6 +73,8 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
+    p.add_argument("--waves", type=int, default=0,
+                   help="waves per workgroup of the batched fitter (0: the kernel's default)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
                    help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
     p.add_argument("--serve-loads", default="0.05,0.1,0.2",
@@ -186,6 +188,8 @@ def run_batched(precision, args, images, backgrounds, rows, counts_np):
     F, S, N = args.frames, args.size, args.emitters
     bf = dao_batch.BatchFitter([(S, S)] * F, [N] * F, model="Z", roi_size=ROI, margin=args.margin,
                                precision=precision, **zkw())
+    if args.waves:
+        bf.setTuning(args.waves, -1, -1)
 
     def step():
         bf.newImages(images)

@@ -110,8 +110,9 @@ void daoBatchRemoveErrorPeaks(daoBatch *b);
 /* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
  * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */
 void daoBatchSetTiming(daoBatch *b, int on);
-/* Tuning of the fitter: waves per workgroup (0: the kernel's maximum, 8 for
- * ROIs up to 512 pixels, else 4; fewer waves leave room for more frames per
+/* Tuning of the fitter: waves per workgroup (0: automatic -- the kernel's
+ * maximum, 8 for ROIs up to 512 pixels, else 4, and 4 when a launch fits more
+ * frames than the device has CUs; fewer waves leave room for more frames per
  * CU), the reserved-box margin in pixels (>= 0; -1 keeps it) and the team
  * size: a frame with n peaks is fitted by min(64, ceil(n / team))
  * workgroups on as many CUs (0: one workgroup per frame; -1 keeps it), which

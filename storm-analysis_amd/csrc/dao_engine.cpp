@@ -54,6 +54,9 @@ static void dfree(T*& p) {
 Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) fail("no HIP device available");
+  int dev = 0;
+  DAO_CHECK(hipGetDevice(&dev));
+  DAO_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
   DAO_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   const char* ex = std::getenv("DAO_EXECUTOR");
   use_rounds = ex != nullptr && std::strcmp(ex, "rounds") == 0;
@@ -454,6 +457,9 @@ int Engine::fit_frames(int max_more) {
   }
   FitArgs a;
   a.m = m;
+  // More frames than CUs: four waves per workgroup, so that two frames share
+  // a CU and the batch runs in one round (eight waves per frame otherwise).
+  if (a.m.waves == 0 && nblocks == 0 && (int)list.size() > num_cus) a.m.waves = 4;
   a.px = px;
   a.pk = pk;
   a.frames = d_fr;

@@ -95,6 +95,7 @@ class Engine {
   size_t istage_bytes = 0;
   int max_attempts_seen = 0;
   long long escapes_total = 0;
+  int num_cus = 0;  // compute units of the device
   long long rounds_total = 0;
 
   void set_model(const Model& mm);
