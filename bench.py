@@ -48,10 +48,10 @@ ALG_BYTES_PER_PIXEL = 36
 # --copy-calibration 1024`: FETCH_SIZE reads half the bytes moved, WRITE_SIZE
 # all of them).
 MEASURED_TRAFFIC = {
-    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
-              "fetch_kib": 95332820.6, "write_kib": 121726713.1, "fetch_scale": 2.0, "write_scale": 1.0},
-    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2,
-             "fetch_kib": 116015976.5, "write_kib": 154402598.2, "fetch_scale": 2.0, "write_scale": 1.0},
+    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0,
+              "fetch_kib": 95110555.8, "write_kib": 102390141.1, "fetch_scale": 2.0, "write_scale": 1.0},
+    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0,
+             "fetch_kib": 116141474.5, "write_kib": 140691942.9, "fetch_scale": 2.0, "write_scale": 1.0},
 }
 # accuracy bar of mixed precision against FP64 for x and y
 TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
@@ -221,7 +221,8 @@ def run_batched(precision, args, images, backgrounds, rows, counts_np):
     achieved = alg_bytes / (lm_ms * 1e-3) / 1e9 if lm_ms > 0 else 0.0
     traffic = None
     mt = MEASURED_TRAFFIC.get(precision)
-    if mt and (F, S, N, args.margin) == (mt["frames"], mt["size"], mt["emitters"], mt["margin"]):
+    if mt and (F, S, N, args.margin, args.waves) == (mt["frames"], mt["size"], mt["emitters"], mt["margin"],
+                                                      mt["waves"]):
         traffic = 1024.0 * (mt["fetch_kib"] * mt["fetch_scale"] + mt["write_kib"] * mt["write_scale"])
     st = bf.stats()
     bf.cleanup()
