@@ -341,11 +341,11 @@ __device__ inline void journal_header(double* snap, unsigned stale, int xi, int 
 
 // Writes snapshot snap's foreground, background sums, cached values and
 // stale flags back to the pixels it was taken from (c supplies the ROI's
-// pixel offsets).
-template <int PPL, class M, class R>
+// pixel offsets).  M accesses the pixels, JM the snapshot.
+template <int PPL, class M, class JM, class R>
 __device__ void journal_restore(const Pix& px, const Frame& fr, const R& c, const double* snap) {
   const int l = lane_id();
-  const long long h = M::ld(reinterpret_cast<const long long*>(snap) + 3 * PPL * kWave + l);
+  const long long h = JM::ld(reinterpret_cast<const long long*>(snap) + 3 * PPL * kWave + l);
   const unsigned stale = (unsigned)h;
   const int org = (int)(h >> 32);
   const long long base = fr.pix0 + (long long)(org >> 16) * fr.W + (org & 0xffff);
@@ -353,23 +353,30 @@ __device__ void journal_restore(const Pix& px, const Frame& fr, const R& c, cons
   for (int s = 0; s < PPL; s++) {
     if (c.ok(s)) {
       PixRW* q = px.rw + base + c.koff[s];
-      M::st(&q->f, M::ld(snap + s * kWave + l));
-      M::st(&q->bg, M::ld(snap + (PPL + s) * kWave + l));
-      M::st(&q->tfi, M::ld(snap + (2 * PPL + s) * kWave + l));
+      M::st(&q->f, JM::ld(snap + s * kWave + l));
+      M::st(&q->bg, JM::ld(snap + (PPL + s) * kWave + l));
+      M::st(&q->tfi, JM::ld(snap + (2 * PPL + s) * kWave + l));
       M::st(&q->stale, (int)((stale >> s) & 1u));
     }
   }
 }
 
 // Puts back, newest first, the nsnap snapshots of journal jr (snapshot 0 in
-// `stage` instead when given): every pixel ends with the state its first
-// snapshot holds (counts aside).
+// the wave's LDS `stage` instead when given): every pixel ends with the
+// state its first snapshot holds (counts aside).
 template <int PPL, class M, class R>
 __device__ void journal_unwind(const Pix& px, const Frame& fr, const R& c, const double* jr, int nsnap,
                                const double* stage = nullptr) {
   for (int v = nsnap - 1; v >= 0; v--) {
     dev_order();  // the previous writes of these pixels have completed
-    journal_restore<PPL, M>(px, fr, c, (v == 0 && stage) ? stage : jr + v * journal_snap_words(PPL));
+    const double* snap = (v == 0 && stage) ? stage : jr + v * journal_snap_words(PPL);
+    if constexpr (std::is_same<M, MemL1>::value) {
+      journal_restore<PPL, M, M>(px, fr, c, snap);
+    } else if (v == 0 && stage) {
+      journal_restore<PPL, M, MemL1>(px, fr, c, snap);
+    } else {
+      journal_restore<PPL, M, M>(px, fr, c, snap);
+    }
   }
 }
 
@@ -427,9 +434,9 @@ struct Roi {
     valid = false;
   }
 
-  // Loads the ROI at (xi, yi); with snap, also journals the pixels' fit
-  // state there (see kJournalSnaps).
-  template <class M = MemL1>
+  // Loads the ROI at (xi, yi) (pixels through M); with snap, also journals
+  // the pixels' fit state there (see kJournalSnaps; the snapshot through JM).
+  template <class M = MemL1, class JM = M>
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
@@ -454,8 +461,8 @@ struct Roi {
     if (snap) {
 #pragma unroll
       for (int s = 0; s < PPL; s++)
-        if (ok(s)) journal_pixel<PPL, M>(snap, s, f[s], bg[s], tfi[s]);
-      journal_header<PPL, M>(snap, stale, xi, yi);
+        if (ok(s)) journal_pixel<PPL, JM>(snap, s, f[s], bg[s], tfi[s]);
+      journal_header<PPL, JM>(snap, stale, xi, yi);
     }
     at_x = xi;
     at_y = yi;
@@ -802,7 +809,7 @@ struct RoiM {
     valid = false;
   }
 
-  template <class M = MemL1>
+  template <class M = MemL1, class JM = M>
   __device__ void load(const Pix& px, const Frame& fr, int xi, int yi, double* snap = nullptr) {
     base = fr.pix0 + (long long)yi * fr.W + xi;
     stale = 0;
@@ -829,8 +836,8 @@ struct RoiM {
     if (snap) {
 #pragma unroll
       for (int s = 0; s < PPL; s++)
-        if (ok(s)) journal_pixel<PPL, M>(snap, s, f[s], bg[s], tfi[s]);
-      journal_header<PPL, M>(snap, stale, xi, yi);
+        if (ok(s)) journal_pixel<PPL, JM>(snap, s, f[s], bg[s], tfi[s]);
+      journal_header<PPL, JM>(snap, stale, xi, yi);
     }
     at_x = xi;
     at_y = yi;
@@ -1198,7 +1205,13 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
 
   R c;
   c.init(m, roi_tab, fr, slice);
-  c.template load<M>(px, fr, w.xi, w.yi, (jr && stage) ? stage : jr);
+  if constexpr (std::is_same<M, MemL1>::value) {
+    c.template load<M>(px, fr, w.xi, w.yi, (jr && stage) ? stage : jr);
+  } else if (jr && stage) {
+    c.template load<M, MemL1>(px, fr, w.xi, w.yi, stage);  // the first snapshot into LDS
+  } else {
+    c.template load<M>(px, fr, w.xi, w.yi, jr);
+  }
   if (jr) nsnap = 1;
   c.shape(m, w);
 
@@ -1715,6 +1728,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
         break;
       }
       int done = 1 + nsnap;
+      bool copy = false;
       if (jr && stage) {
         // The staged snapshot goes to the op's journal if an earlier op is
         // still running or has left its box (the op may then be undone);
@@ -1726,12 +1740,8 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
           const int c = __hip_atomic_load(&sh.cur[lane], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
           open = c >= 0 && c < j;
         }
-        open = __any(open) || __hip_atomic_load(&sh.esc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < j;
-        if (open) {
-          for (int q = lane; q < (int)journal_snap_words(PPL); q += kWave) MemL1::st_stream(jr + q, stage[q]);
-        } else {
-          done = 1;  // never undone: no journal
-        }
+        copy = __any(open) || __hip_atomic_load(&sh.esc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < j;
+        if (!copy) done = 1;  // never undone: no journal
       }
       if (lane == 0) {
 #pragma unroll
@@ -1743,6 +1753,10 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       fpw_set_cur(&sh.cur[wv], kIdle);
+      // The journal is read only by a rollback, after the workgroup's
+      // barrier: the copy need not hold back the ops waiting for this one.
+      if (copy)
+        for (int q = lane; q < (int)journal_snap_words(PPL); q += kWave) MemL1::st_stream(jr + q, stage[q]);
     }
     __syncthreads();
     const int e = sh.esc;
@@ -1988,9 +2002,10 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
 #pragma unroll
     for (int q = 0; q < C_COUNT; q++) d[q] = 0;
     int r = 1, nsnap = 0;
+    double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
+    double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
     if (kind == OP_LM) {
-      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d,
-                                         op_journal<PPL>(a, fr, slot, b), nsnap);
+      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
     } else {
       recenter_turn<PPL, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
     }
@@ -2010,6 +2025,11 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
       MemDev::st(&a.op_done[slot], 1 + nsnap);
       MemDev::add(&ctl->done, 1);
     }
+    // The staged first snapshot goes to the journal after the op is
+    // published: only a rollback reads it, once every seat is idle (a seat
+    // is left after dev_order).
+    if (jr && stage)
+      for (int q = lane; q < (int)journal_snap_words(PPL); q += kWave) MemDev::st(jr + q, stage[q]);
   }
   dev_order();
   if (lane == 0) MemDev::st(&a.team_cur[seat], kIdle);
@@ -2151,7 +2171,7 @@ __device__ __noinline__ void team_transition(const FitArgs& a, const Frame& fr, 
 template <int PPL, int SHAPE, int NW, bool MP>
 __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
   static_assert(NW <= kTeamSeats, "seats per workgroup");
-  constexpr int kSlice = fit_frames_slice<PPL, MP>();
+  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
   __shared__ TeamShared sh;
   extern __shared__ double lds_dyn[];
   double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
@@ -2241,7 +2261,7 @@ void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
 template <int PPL, int SHP, int NW, bool MP>
 void launch_fit_team_t(const FitArgs& a, int nblocks, hipStream_t st) {
   const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
-  const size_t bytes = sizeof(double) * (size_t)nw * fit_frames_slice<PPL, MP>();
+  const size_t bytes = sizeof(double) * (size_t)nw * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;
   if (bytes > allowed) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_team<PPL, SHP, NW, MP>),
