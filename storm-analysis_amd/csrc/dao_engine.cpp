@@ -145,6 +145,7 @@ Engine::~Engine() {
   dfree(d_sweeps);
   dfree(d_flist);
   dfree(d_err);
+  dfree(d_sweep_rw);
   dfree(d_team_ctl);
   dfree(d_team_cur);
   dfree(d_team_tab);
@@ -186,6 +187,7 @@ void Engine::alloc_sched(long long c) {
   d_flag = dalloc<int>(c);
   d_run = dalloc<int>(c);
   d_journal = dalloc<double>(c * journal_words(ppl_for(m.nroi)));
+  d_sweep_rec = dalloc<PeakRec>(c);
 }
 
 void Engine::free_sched() {
@@ -203,6 +205,7 @@ void Engine::free_sched() {
   dfree(d_flag);
   dfree(d_run);
   dfree(d_journal);
+  dfree(d_sweep_rec);
 }
 
 // Grow every per-slot array to c slots, keeping the first `cap` records.
@@ -529,6 +532,11 @@ int Engine::sweep() {
   const int started = sc.nops;
   if (started == 0) return 0;
   sweeps_total++;
+  // A frame whose turn leaves its box returns to its state at the sweep's
+  // start (exact) and reruns the sweep with that turn alone in the frame.
+  if (d_sweep_rw == nullptr) d_sweep_rw = dalloc<PixRW>(npix);
+  DAO_CHECK(hipMemcpyAsync(d_sweep_rw, px.rw, sizeof(PixRW) * (size_t)npix, hipMemcpyDeviceToDevice, st));
+  DAO_CHECK(hipMemcpyAsync(d_sweep_rec, pk.rec, sizeof(PeakRec) * (size_t)cap, hipMemcpyDeviceToDevice, st));
   int attempt = 0;
   while (sc.nops > 0) {
     launch_box(OP_LM, sc, pk, d_fr, m, d_override, nullptr, 0, 0, st);
@@ -542,8 +550,7 @@ int Engine::sweep() {
     for (int f = 0; f < F; f++) bad += h[f] ? 1 : 0;
     if (bad == 0) break;
     escapes_total += bad;
-    OpArgs a = args(0);
-    launch_rollback(a, st);
+    launch_restore_bad_frames(d_fr, F, sc.frame_bad, px.rw, d_sweep_rw, pk.rec, d_sweep_rec, st);
     launch_collect_escapes(sc, pk, d_override, st);
     attempt++;
     max_attempts_seen = std::max(max_attempts_seen, attempt);

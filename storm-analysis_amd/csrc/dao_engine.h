@@ -22,7 +22,6 @@ namespace dao {
 // kernel launchers (dao_kernels.hip)
 
 void launch_op_round(int kind, const OpArgs& a, hipStream_t st, const double* rows, int fmt, int first_slot);
-void launch_rollback(const OpArgs& a, hipStream_t st);
 void launch_binning(const Sched& sc, const Frame* frames, const Peaks& pk, int ncells, hipStream_t st);
 void launch_box(int kind, const Sched& sc, const Peaks& pk, const Frame* frames, const Model& m,
                 const int4* override_box, const double* rows, int fmt, int first_slot, hipStream_t st);
@@ -50,6 +49,8 @@ void launch_iota(int* p, int n, int base, hipStream_t st);
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st);
 void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st);
 void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st);
+void launch_restore_bad_frames(const Frame* frames, int nframes, const int* bad, PixRW* rw, const PixRW* rw_copy,
+                               PeakRec* rec, const PeakRec* rec_copy, hipStream_t st);
 void launch_pack_peaks(const Peaks& pk, const Frame* frames, const int* offset, const Model& m, long long nslots,
                        double* out, int* status, hipStream_t st);
 
@@ -173,6 +174,8 @@ class Engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* d_run = nullptr;  // RUNNING list of the frame fitters (per slot)
   double* d_journal = nullptr;  // LM turn journals of the frame fitters (per slot)
+  PixRW* d_sweep_rw = nullptr;    // multi-kernel executor: pixel fit state at the sweep's start
+  PeakRec* d_sweep_rec = nullptr;  // and the peak records (per slot)
   int* d_sweeps = nullptr;
   int* d_flist = nullptr;
   int* d_err = nullptr;

@@ -203,20 +203,6 @@ __global__ __launch_bounds__(256) void k_round(OpArgs a, int kind, const double*
   op_finish(a.sc, i, a.round, r, d);
 }
 
-// Undo the completed LM ops of frames flagged bad (the ops of one frame may
-// overlap; the undo uses atomic adds, so its rounding order may vary).
-template <int PPL>
-__global__ __launch_bounds__(256) void k_rollback(OpArgs a) {
-  __shared__ double lds[kWavesPerBlock][RoiLds<PPL>::kDoubles];
-  const int wv = threadIdx.x >> 6;
-  const int i = blockIdx.x * kWavesPerBlock + wv;
-  if (i >= a.sc.nops) return;
-  const int p = a.sc.op_peak[i];
-  const int fi = a.pk.rec[p].frame;
-  if (!a.sc.frame_bad[fi] || (a.sc.op_flags[i] & 1) == 0) return;
-  undo_turn<PPL>(a.m, a.px, a.pk, a.frames[fi], a.roi_tab, lds[wv], p, nullptr, 0);
-}
-
 // ---------------------------------------------------------------------------
 // per-pixel and per-peak kernels
 // ---------------------------------------------------------------------------
@@ -576,10 +562,6 @@ void launch_op_round(int kind, const OpArgs& a, hipStream_t st, const double* ro
   }
 }
 
-void launch_rollback(const OpArgs& a, hipStream_t st) {
-  if (a.sc.nops == 0) return;
-  DAO_PPL_SWITCH(a.m.nroi, (k_rollback<PPL><<<waves_grid(a.sc.nops), 256, 0, st>>>(a)));
-}
 
 void launch_binning(const Sched& sc, const Frame* frames, const Peaks& pk, int ncells, hipStream_t st) {
   if (sc.nops == 0) return;
@@ -684,6 +666,32 @@ __global__ void k_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats) {
     ctl[i] = c;
   }
   if (i < nseats) cur[i] = kIdle;
+}
+
+// Frames flagged bad return to the copy of their pixel fit state and peak
+// records taken at the sweep's start: block (f, c) copies chunk c of frame f.
+__global__ void k_restore_bad_frames(const Frame* frames, const int* bad, PixRW* rw, const PixRW* rw_copy,
+                                     PeakRec* rec, const PeakRec* rec_copy) {
+  if (!bad[blockIdx.x]) return;
+  const Frame fr = frames[blockIdx.x];
+  constexpr int kQ = sizeof(PixRW) / sizeof(int4);
+  const long long n = (long long)fr.W * fr.H * kQ;
+  const long long per = (n + gridDim.y - 1) / gridDim.y;
+  const long long lo = per * blockIdx.y, hi = lo + per < n ? lo + per : n;
+  const int4* src = reinterpret_cast<const int4*>(rw_copy + fr.pix0);
+  int4* dst = reinterpret_cast<int4*>(rw + fr.pix0);
+  for (long long i = lo + threadIdx.x; i < hi; i += blockDim.x) dst[i] = src[i];
+  if (blockIdx.y == 0) {
+    constexpr int kR = sizeof(PeakRec) / sizeof(long long);
+    const long long* rs = reinterpret_cast<const long long*>(rec_copy + fr.peak0);
+    long long* rd = reinterpret_cast<long long*>(rec + fr.peak0);
+    for (long long i = threadIdx.x; i < (long long)fr.nfit * kR; i += blockDim.x) rd[i] = rs[i];
+  }
+}
+
+void launch_restore_bad_frames(const Frame* frames, int nframes, const int* bad, PixRW* rw, const PixRW* rw_copy,
+                               PeakRec* rec, const PeakRec* rec_copy, hipStream_t st) {
+  if (nframes > 0) k_restore_bad_frames<<<dim3(nframes, 16), 256, 0, st>>>(frames, bad, rw, rw_copy, rec, rec_copy);
 }
 
 void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st) {

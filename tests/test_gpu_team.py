@@ -14,7 +14,7 @@ import numpy
 import pytest
 
 from test_gpu_parity import TOL_POS, assert_parity, fit_frame
-from test_gpu_batch import assert_frame_parity, oracle_fit, scmos_frames
+from test_gpu_batch import assert_frame_parity, executor, oracle_fit, scmos_frames
 from storm_analysis_amd.sa_library import dao_batch
 from storm_analysis_amd.sa_library import dao_fit_c as D
 from storm_analysis_amd.diagnostics import synth
@@ -143,6 +143,50 @@ def test_high_density_overlap_matches_oracle(size, density, team):
     assert_parity(g, r)
 
 
+def dense_rollback_frame():
+    size, density = 128, 2.0
+    n = int(round(size * size * 0.0256 * density))
+    rng = numpy.random.default_rng(synth.SEED_BASE + 51)
+    return synth.frame_2d_dense(rng, size=size, n=n), n
+
+
+def batch_fit_2d(fr, n, kw, waves=0, margin=-1):
+    bf = dao_batch.BatchFitter([fr.image.shape], [n], model="2d", **kw)
+    try:
+        if waves or margin >= 0:
+            bf.setTuning(waves, margin, 0)
+        bf.newImages([fr.image])
+        bf.newBackgrounds([numpy.full(fr.image.shape, 100.0)])
+        bf.newPeaks([fr.peaks], "hdf5")
+        bf.doFit()
+        return bf.getPeaks(0)
+    finally:
+        bf.cleanup()
+
+
+def assert_flags_and_positions(g, r):
+    numpy.testing.assert_array_equal(g["status"], r["status"])
+    numpy.testing.assert_array_equal(g["iterations"], r["iterations"])
+    ok = r["status"] != D.ERROR
+    for k in ("x", "y"):
+        assert numpy.abs(g[k] - r[k])[ok].max() < TOL_POS, k
+
+
+@pytest.mark.gpu
+def test_high_density_rounds_executor_matches_oracle():
+    """
+    The multi-kernel executor on the 2.0 emitters/um^2 frame: a frame whose
+    turn leaves its box returns to the copy of its state taken at the
+    sweep's start and reruns the sweep with that turn alone in the frame.
+    """
+    fr, n = dense_rollback_frame()
+    kw = {"roi_size": 18, "sigma_range": [0.96, 8.0]}
+    r = fit_frame("oracle", D.MultiFitter2D, kw, fr.image, fr.peaks)
+    with executor("rounds"):
+        g = batch_fit_2d(fr, n, kw)
+    assert_flags_and_positions(g, r)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("waves,margin", [(1, 20), (8, 2), (8, 20), (2, 2)])
 def test_high_density_rollbacks_match_oracle(waves, margin):
@@ -155,24 +199,8 @@ def test_high_density_rollbacks_match_oracle(waves, margin):
     removed again, so only an exact restore of the pixels keeps the
     reference's results -- whatever the waves per workgroup and the margin.
     """
-    size, density = 128, 2.0
-    n = int(round(size * size * 0.0256 * density))
-    rng = numpy.random.default_rng(synth.SEED_BASE + 51)
-    fr = synth.frame_2d_dense(rng, size=size, n=n)
+    fr, n = dense_rollback_frame()
     kw = {"roi_size": 18, "sigma_range": [0.96, 8.0]}
     r = fit_frame("oracle", D.MultiFitter2D, kw, fr.image, fr.peaks)
-    bf = dao_batch.BatchFitter([fr.image.shape], [n], model="2d", **kw)
-    try:
-        bf.setTuning(waves, margin, 0)
-        bf.newImages([fr.image])
-        bf.newBackgrounds([numpy.full(fr.image.shape, 100.0)])
-        bf.newPeaks([fr.peaks], "hdf5")
-        bf.doFit()
-        g = bf.getPeaks(0)
-    finally:
-        bf.cleanup()
-    numpy.testing.assert_array_equal(g["status"], r["status"])
-    numpy.testing.assert_array_equal(g["iterations"], r["iterations"])
-    ok = r["status"] != D.ERROR
-    for k in ("x", "y"):
-        assert numpy.abs(g[k] - r[k])[ok].max() < TOL_POS, k
+    g = batch_fit_2d(fr, n, kw, waves, margin)
+    assert_flags_and_positions(g, r)
