@@ -73,6 +73,7 @@ def parse():
     p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
+    p.add_argument("--no-ingest", action="store_true", help="skip the frame-ingest leg")
     p.add_argument("--waves", type=int, default=0,
                    help="waves per workgroup of the batched fitter (0: the kernel's default)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
@@ -367,6 +368,48 @@ def latency_leg(args, images, backgrounds, rows, counts_np, log):
     return out
 
 
+def ingest_leg(log, frames=500, size=512, pad=10, reps=10, cpu_frames=20, seed=20261211):
+    """
+    Frame ingest (daoIngestFrames): 500 big-endian uint16 frames of 512x512,
+    resident on the GPU, to photo-electrons (standard camera: gain 2 ADU/e-,
+    offset 100) mirror-padded by 10 px, as FrameReader.loadAFrame + padArray
+    prepare each frame on the host in the reference.  HBM bound: 2 B read per
+    frame pixel, 8 B written per padded pixel.
+    """
+    import torch
+    from storm_analysis_amd.sa_library import dao_ingest
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import ingest_oracle
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    raw = torch.randint(90, 4000, (frames, size, size), dtype=torch.int32, device="cuda",
+                        generator=g).to(torch.uint16)
+    reader = dao_ingest.FrameReaderStd(camera_gain=2.0, camera_offset=100.0)
+    out = reader.loadFrames(raw, pad=pad, big_endian=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = reader.loadFrames(raw, pad=pad, big_endian=True)
+    dt = (time.perf_counter() - t0) / reps
+    bytes_ = frames * size * size * 2 + frames * (size + 2 * pad) ** 2 * 8
+    host_raw = raw[:cpu_frames].cpu().numpy()
+    t1 = time.perf_counter()
+    want = ingest_oracle.ingest(host_raw, reader.offset, reader.gain, reader.rqe, pad, True)
+    dt_cpu = time.perf_counter() - t1
+    exact = bool(numpy.array_equal(out[:cpu_frames].cpu().numpy(), want))
+    gbs = bytes_ / dt / 1e9
+    log("ingest: %d frames %dx%d -> padded by %d: %.3f ms per call (%.0f GB/s, %.1f%% of HBM), "
+        "bit-identical to the oracle: %s; CPU (numpy, 1 thread) %.1f frames/s" %
+        (frames, size, size, pad, dt * 1e3, gbs, 100 * gbs / HBM_PEAK_GBS, exact, cpu_frames / dt_cpu))
+    return {"frames": frames, "frame_size": size, "pad": pad, "big_endian": True, "ms_per_call": dt * 1e3,
+            "frames_per_s": frames / dt, "bytes_per_call": bytes_,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS},
+            "bit_identical_to_oracle": exact,
+            "cpu_baseline": {"value": cpu_frames / dt_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
+                             "sample": "%d frames through the numpy restatement of loadAFrame + padArray" %
+                                       cpu_frames}}
+
+
 def dense_leg(log, seed=20261210):
     """
     High-density overlap stress: one 2048^2 frame with 20000 emitters
@@ -546,6 +589,8 @@ def main():
     if args.serve_seconds > 0:
         serving = serve_leg(args, head, log)
 
+    ingest = None if args.no_ingest else ingest_leg(log)
+
     hm = meas[head]
     line = {
         "metric": "converged emitter fits/sec + p99 frame latency; batched vs plain, 1 MI355X",
@@ -578,6 +623,7 @@ def main():
         "uneven_batch": uneven,
         "dense_2048": dense,
         "serving": serving,
+        "ingest": ingest,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }
