@@ -74,6 +74,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--margin", type=int, default=2)
     p.add_argument("--no-ingest", action="store_true", help="skip the frame-ingest leg")
+    p.add_argument("--no-filter", action="store_true", help="skip the post-fit filter leg")
     p.add_argument("--waves", type=int, default=0,
                    help="waves per workgroup of the batched fitter (0: the kernel's default)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
@@ -410,6 +411,107 @@ def ingest_leg(log, frames=500, size=512, pad=10, reps=10, cpu_frames=20, seed=2
                                        cpu_frames}}
 
 
+def filter_leg(log, frames=500, per=1000, new_per=100, size=512, sigma=1.5, reps=20, cpu_frames=100,
+               seed=20261212):
+    """
+    Post-fit filters over a batch of peak lists resident on the GPU: 500
+    frames x 1000 peaks uniform in 512 x 512, as fitting.py:718-782 runs them
+    after each fit pass: runningIfHasNeighbors against 100 new peaks per frame,
+    then markDimmerPeaks (r_removal = sigma 1.5, r_neighbors = 5 sigma) and
+    markLowSignificancePeaks (minimum significance 1.5).  One pass = the three
+    batched calls (each synchronous, so launch latency is included) plus the
+    copy that restores the statuses.  Bytes per pass: x, y, status in and out
+    for every peak in each of the three calls, heights or significances for
+    two, and x, y of the new peaks.
+    """
+    import ctypes
+    import torch
+    from storm_analysis_amd.sa_library import dao_filter as DF
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import filter_oracle as FO
+    rng = numpy.random.default_rng(seed)
+    n, nn = frames * per, frames * new_per
+    host = {"x": rng.uniform(0, size, n), "y": rng.uniform(0, size, n), "h": rng.uniform(50.0, 500.0, n),
+            "sig": rng.uniform(0.0, 10.0, n), "nx": rng.uniform(0, size, nn), "ny": rng.uniform(0, size, nn),
+            "status": rng.choice([FO.RUNNING, FO.CONVERGED, FO.CONVERGED, FO.CONVERGED], n).astype(numpy.int32)}
+    dev = {k: torch.from_numpy(v).cuda() for k, v in host.items()}
+    off = torch.arange(0, n + 1, per, dtype=torch.int32, device="cuda")
+    noff = torch.arange(0, nn + 1, new_per, dtype=torch.int32, device="cuda")
+    removed = torch.zeros(frames, dtype=torch.int32, device="cuda")
+    st = dev["status"].clone()
+    lib = DF.loadFilterC()
+    p = {k: ctypes.c_void_p(v.data_ptr()) for k, v in dev.items()}
+    ps, po, pn, pr = (ctypes.c_void_p(t.data_ptr()) for t in (st, off, noff, removed))
+    r_nb = 5.0 * sigma
+
+    def one_pass():
+        st.copy_(dev["status"])
+        torch.cuda.synchronize()
+        rc = lib.daoBatchRunningIfHasNeighbors(p["x"], p["y"], po, p["nx"], p["ny"], pn, ps, frames, r_nb)
+        rc |= lib.daoBatchMarkDimmerPeaks(p["x"], p["y"], p["h"], ps, po, frames, sigma, r_nb, pr)
+        n_dim = int(removed.sum())
+        rc |= lib.daoBatchMarkLowSignificancePeaks(p["x"], p["y"], p["sig"], ps, po, frames, 1.5, r_nb, pr)
+        if rc != 0:
+            raise RuntimeError("filter call failed")
+        return n_dim
+    one_pass()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        one_pass()
+    dt = (time.perf_counter() - t0) / reps
+    got = st.cpu().numpy()
+
+    def cpu_frame(f, impl):
+        a, b = f * per, (f + 1) * per
+        s = host["status"][a:b].copy()
+        x, y = host["x"][a:b].copy(), host["y"][a:b].copy()
+        nx, ny = host["nx"][f * new_per:(f + 1) * new_per].copy(), host["ny"][f * new_per:(f + 1) * new_per].copy()
+        h, sig = host["h"][a:b].copy(), host["sig"][a:b].copy()
+        if impl is None:
+            FO.running_if_has_neighbors(s, x, y, nx, ny, r_nb)
+            FO.mark_dimmer_peaks(x, y, h, s, sigma, r_nb)
+            FO.mark_low_significance_peaks(x, y, sig, s, 1.5, r_nb)
+        else:
+            impl.runningIfHasNeighbors(x, y, nx, ny, s, r_nb, per, new_per)
+            impl.markDimmerPeaks(x, y, h, s, sigma, r_nb, per)
+            impl.markLowSignificancePeaks(x, y, sig, s, 1.5, r_nb, per)
+        return s
+    exact = all(numpy.array_equal(got[f * per:(f + 1) * per], cpu_frame(f, None)) for f in (0, frames // 2, frames - 1))
+    ref_so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "libia_utilities.so")
+    impl, kind, what = None, "port", "the numpy restatement (oracle/filter_oracle.py)"
+    if os.path.exists(ref_so):
+        impl = ctypes.cdll.LoadLibrary(ref_so)
+        P = numpy.ctypeslib.ndpointer
+        d, i32 = P(dtype=numpy.float64), P(dtype=numpy.int32)
+        for name in ("markDimmerPeaks", "markLowSignificancePeaks"):
+            getattr(impl, name).argtypes = [d, d, d, i32, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+            getattr(impl, name).restype = ctypes.c_int
+        impl.runningIfHasNeighbors.argtypes = [d, d, d, d, i32, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        impl.runningIfHasNeighbors.restype = None
+        kind, what = "reference", "the reference's ia_utilities.c + kdtree.c built from source (oracle/_ref)"
+    t1 = time.perf_counter()
+    for f in range(cpu_frames):
+        s = cpu_frame(f, impl)
+        if impl is not None and f in (0, frames // 2):
+            exact = exact and numpy.array_equal(got[f * per:(f + 1) * per], s)
+    dt_cpu = (time.perf_counter() - t1) / cpu_frames
+    bytes_ = n * (3 * 16 + 3 * 8 + 2 * 8) + nn * 16
+    gbs = bytes_ / dt / 1e9
+    log("filters: %d frames x %d peaks: %.3f ms per pass (runningIfHasNeighbors + markDimmerPeaks + "
+        "markLowSignificancePeaks, %.1f M peaks/s, %.2f%% of HBM), statuses identical to the oracle%s: %s; "
+        "CPU (%s, 1 thread) %.2f ms per frame" %
+        (frames, per, dt * 1e3, n / dt / 1e6, 100 * gbs / HBM_PEAK_GBS,
+         " and the reference C" if impl is not None else "", exact, kind, dt_cpu * 1e3))
+    return {"frames": frames, "peaks_per_frame": per, "new_peaks_per_frame": new_per, "r_removal": sigma,
+            "r_neighbors": r_nb, "min_significance": 1.5, "ms_per_pass": dt * 1e3, "peaks_per_s": n / dt,
+            "bytes_per_pass": bytes_,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS},
+            "identical_to_oracle": bool(exact),
+            "cpu_baseline": {"value": per / dt_cpu, "unit": "peaks/s", "cores": 1, "kind": kind,
+                             "sample": "%d frames of %d peaks through %s" % (cpu_frames, per, what)}}
+
+
 def dense_leg(log, seed=20261210):
     """
     High-density overlap stress: one 2048^2 frame with 20000 emitters
@@ -590,6 +692,7 @@ def main():
         serving = serve_leg(args, head, log)
 
     ingest = None if args.no_ingest else ingest_leg(log)
+    filters = None if args.no_filter else filter_leg(log)
 
     hm = meas[head]
     line = {
@@ -624,6 +727,7 @@ def main():
         "dense_2048": dense,
         "serving": serving,
         "ingest": ingest,
+        "filters": filters,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }

@@ -15,7 +15,8 @@
 // runningIfHasNeighbors is pass B alone, with the new peaks as the searched set.
 //
 // Neighbour search: a uniform grid over the frame's searched peaks, built in
-// LDS per call (bounding box, counting sort of peak indices by cell), with
+// LDS per call (bounding box, counting sort of the peaks by cell, their
+// positions and values staged in LDS for frames of up to 2048 peaks), with
 // cells at least 1.001 x the largest radius wide, so every peak within the
 // radius of a query lies in the 3 x 3 cells around it.  The distance test is
 // the kd-tree's, operation for operation: (x_k - x_i)^2 + (y_k - y_i)^2 <= r^2
@@ -34,6 +35,7 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kGridMax = 64;  // cells per axis
 constexpr int kCellsMax = kGridMax * kGridMax;
+constexpr int kLdsPeaks = 2048;  // frames up to this many searched peaks are staged in LDS
 constexpr int32_t kRunning = 0, kConverged = 1, kError = 2;
 
 enum Mode { kDimmer = 0, kLowSig = 1, kRunningIf = 2 };
@@ -104,10 +106,47 @@ __device__ inline int cell_of(const Grid& g, double x, double y) {
   return cy * g.gx + cx;
 }
 
-// Bounding box, cell sizes and the counting sort of the searched peaks
-// t0 .. t1-1 into a.sorted[t0 ..], cell c holding cell[c] .. cell[c+1]-1.
-__device__ Grid build_grid(const FilterArgs& a, int t0, int t1, double rmax, int* cell, double* s_d, int* s_i) {
+// A frame's searched peaks t0 .. t0+n-1 in search order s = 0 .. n-1: cell by
+// cell after the counting sort (peak order for the brute-force search).  A
+// frame of at most kLdsPeaks peaks on the grid is staged in LDS (position,
+// value, peak index, removed flag per s), so the neighbour scans read LDS
+// only; a larger one is read through a.sorted from global memory.
+struct Frame {
+  const FilterArgs* a;
+  double* sx;
+  double* sy;
+  double* sv;
+  int* sk;
+  uint8_t* sf;
+  int t0, n;
+  bool lds, brute;
+  __device__ int peak(int s) const { return lds ? sk[s] : (brute ? t0 + s : a->sorted[t0 + s]); }
+  __device__ double v(int s) const { return lds ? sv[s] : a->val[peak(s)]; }
+  __device__ bool flag(int s) const { return lds ? sf[s] != 0 : a->flag[peak(s)] != 0; }
+  __device__ void set_flag(int s, bool b) const {
+    if (lds)
+      sf[s] = b ? 1 : 0;
+    else
+      a->flag[peak(s)] = b ? 1 : 0;
+  }
+  __device__ void pos(int s, double& x, double& y) const {
+    if (lds) {
+      x = sx[s];
+      y = sy[s];
+    } else {
+      const int k = peak(s);
+      x = a->tx[k];
+      y = a->ty[k];
+    }
+  }
+};
+
+// Bounding box, cell sizes and the counting sort of the searched peaks into
+// search order, cell c holding s = cell[c] .. cell[c+1]-1.
+__device__ Grid build_grid(const FilterArgs& a, Frame& fr, double rmax, bool with_val, int* cell, double* s_d,
+                           int* s_i) {
   const int w = threadIdx.x >> 6;
+  const int t0 = fr.t0, t1 = fr.t0 + fr.n;
   double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
   int bad = 0;
   for (int k = t0 + threadIdx.x; k < t1; k += kThreads) {
@@ -144,6 +183,8 @@ __device__ Grid build_grid(const FilterArgs& a, int t0, int t1, double rmax, int
   __syncthreads();
   const double wx = xmax - xmin, wy = ymax - ymin;
   g.brute = bad != 0 || !(rmax > 0.0) || !isfinite(rmax) || !isfinite(wx) || !isfinite(wy);
+  fr.brute = g.brute;
+  fr.lds = !g.brute && fr.n <= kLdsPeaks;
   if (g.brute) return g;
   g.x0 = xmin;
   g.y0 = ymin;
@@ -166,25 +207,35 @@ __device__ Grid build_grid(const FilterArgs& a, int t0, int t1, double rmax, int
     acc += cell[c];
     cell[c] = acc;
   }
-  if (threadIdx.x == 0) cell[nc] = t1 - t0;
+  if (threadIdx.x == 0) cell[nc] = fr.n;
   __syncthreads();
   // scatter: counting down leaves cell[c] at the start of cell c
   for (int k = t0 + threadIdx.x; k < t1; k += kThreads) {
-    const int pos = atomicSub(&cell[cell_of(g, a.tx[k], a.ty[k])], 1) - 1;
-    a.sorted[t0 + pos] = k;
+    const double x = a.tx[k], y = a.ty[k];
+    const int s = atomicSub(&cell[cell_of(g, x, y)], 1) - 1;
+    if (fr.lds) {
+      fr.sk[s] = k;
+      fr.sx[s] = x;
+      fr.sy[s] = y;
+      if (with_val) fr.sv[s] = a.val[k];
+    } else {
+      a.sorted[t0 + s] = k;
+    }
   }
   __syncthreads();
   return g;
 }
 
-// True when some searched peak k (t0 <= k < t1) with pred(k) lies within
-// sqrt(r2) of (x, y).
+// True when some searched peak s with pred(s) lies within sqrt(r2) of (x, y).
 template <class Pred>
-__device__ inline bool any_near(const FilterArgs& a, const Grid& g, const int* cell, int t0, int t1, double x,
-                                double y, double r2, Pred pred) {
+__device__ inline bool any_near(const Frame& fr, const Grid& g, const int* cell, double x, double y, double r2,
+                                Pred pred) {
   if (g.brute) {
-    for (int k = t0; k < t1; ++k)
-      if (dist_sq(a.tx[k], a.ty[k], x, y) <= r2 && pred(k)) return true;
+    for (int s = 0; s < fr.n; ++s) {
+      double px, py;
+      fr.pos(s, px, py);
+      if (dist_sq(px, py, x, y) <= r2 && pred(s)) return true;
+    }
     return false;
   }
   if (!isfinite(x) || !isfinite(y)) return false;  // no distance from it is <= r2
@@ -196,8 +247,9 @@ __device__ inline bool any_near(const FilterArgs& a, const Grid& g, const int* c
   for (int cy = ylo; cy <= yhi; ++cy) {
     const int s1 = cell[cy * g.gx + xhi + 1];
     for (int s = cell[cy * g.gx + xlo]; s < s1; ++s) {
-      const int k = a.sorted[t0 + s];
-      if (dist_sq(a.tx[k], a.ty[k], x, y) <= r2 && pred(k)) return true;
+      double px, py;
+      fr.pos(s, px, py);
+      if (dist_sq(px, py, x, y) <= r2 && pred(s)) return true;
     }
   }
   return false;
@@ -206,6 +258,9 @@ __device__ inline bool any_near(const FilterArgs& a, const Grid& g, const int* c
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
   __shared__ int cell[kCellsMax + 1];
+  __shared__ double s_x[kLdsPeaks], s_y[kLdsPeaks], s_v[MODE == kRunningIf ? 1 : kLdsPeaks];
+  __shared__ int s_k[kLdsPeaks];
+  __shared__ uint8_t s_f[MODE == kRunningIf ? 1 : kLdsPeaks];
   __shared__ double s_d[4 * kWaves];
   __shared__ int s_i[kWaves];
   const int f = blockIdx.x;
@@ -215,43 +270,51 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     if (MODE != kRunningIf && threadIdx.x == 0) a.removed[f] = 0;
     return;
   }
+  Frame fr{&a, s_x, s_y, s_v, s_k, s_f, t0, t1 - t0, false, false};
   if (MODE == kRunningIf) {
-    const Grid g = build_grid(a, t0, t1, fabs(a.ra), cell, s_d, s_i);
+    const Grid g = build_grid(a, fr, fabs(a.ra), false, cell, s_d, s_i);
     const double r2 = __dmul_rn(a.ra, a.ra);
     for (int i = q0 + threadIdx.x; i < q1; i += kThreads) {
       const int32_t st = a.status[i];
       if (st == kRunning || st == kError) continue;
-      if (any_near(a, g, cell, t0, t1, a.qx[i], a.qy[i], r2, [](int) { return true; })) a.status[i] = kRunning;
+      if (any_near(fr, g, cell, a.qx[i], a.qy[i], r2, [](int) { return true; })) a.status[i] = kRunning;
     }
     return;
   }
   // pass A: flag the peaks the reference removes
-  const Grid g = build_grid(a, t0, t1, MODE == kDimmer ? fmax(fabs(a.ra), fabs(a.rb)) : fabs(a.rb), cell, s_d, s_i);
+  const Grid g = build_grid(a, fr, MODE == kDimmer ? fmax(fabs(a.ra), fabs(a.rb)) : fabs(a.rb), MODE == kDimmer,
+                            cell, s_d, s_i);
   if (MODE == kDimmer) {
     const double r2 = __dmul_rn(a.ra, a.ra);
-    for (int i = t0 + threadIdx.x; i < t1; i += kThreads) {
+    for (int s = threadIdx.x; s < fr.n; s += kThreads) {
       bool rm = false;
-      if (a.status[i] != kError) {
-        const double hi = a.val[i];
-        rm = any_near(a, g, cell, t0, t1, a.tx[i], a.ty[i], r2, [&](int k) { return a.val[k] > hi; });
+      if (a.status[fr.peak(s)] != kError) {
+        const double hs = fr.v(s);
+        double x, y;
+        fr.pos(s, x, y);
+        rm = any_near(fr, g, cell, x, y, r2, [&](int j) { return fr.v(j) > hs; });
       }
-      a.flag[i] = rm ? 1 : 0;
+      fr.set_flag(s, rm);
     }
   } else {
-    for (int i = t0 + threadIdx.x; i < t1; i += kThreads)
-      a.flag[i] = (a.status[i] != kError && !(a.val[i] > a.ra)) ? 1 : 0;
+    for (int s = threadIdx.x; s < fr.n; s += kThreads) {
+      const int k = fr.peak(s);
+      fr.set_flag(s, a.status[k] != kError && !(a.val[k] > a.ra));
+    }
   }
   __syncthreads();
   // pass B: removed -> ERROR; CONVERGED near a removed peak -> RUNNING
   const double r2 = __dmul_rn(a.rb, a.rb);
   int n_removed = 0;
-  for (int i = t0 + threadIdx.x; i < t1; i += kThreads) {
-    if (a.flag[i]) {
-      a.status[i] = kError;
+  for (int s = threadIdx.x; s < fr.n; s += kThreads) {
+    const int k = fr.peak(s);
+    if (fr.flag(s)) {
+      a.status[k] = kError;
       ++n_removed;
-    } else if (a.status[i] == kConverged &&
-               any_near(a, g, cell, t0, t1, a.tx[i], a.ty[i], r2, [&](int k) { return a.flag[k] != 0; })) {
-      a.status[i] = kRunning;
+    } else if (a.status[k] == kConverged) {
+      double x, y;
+      fr.pos(s, x, y);
+      if (any_near(fr, g, cell, x, y, r2, [&](int j) { return fr.flag(j); })) a.status[k] = kRunning;
     }
   }
   n_removed = wave_sum(n_removed);
