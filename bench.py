@@ -75,6 +75,7 @@ def parse():
     p.add_argument("--margin", type=int, default=2)
     p.add_argument("--no-ingest", action="store_true", help="skip the frame-ingest leg")
     p.add_argument("--no-filter", action="store_true", help="skip the post-fit filter leg")
+    p.add_argument("--no-finder", action="store_true", help="skip the local-maxima finder leg")
     p.add_argument("--waves", type=int, default=0,
                    help="waves per workgroup of the batched fitter (0: the kernel's default)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
@@ -512,6 +513,90 @@ def filter_leg(log, frames=500, per=1000, new_per=100, size=512, sigma=1.5, reps
                              "sample": "%d frames of %d peaks through %s" % (cpu_frames, per, what)}}
 
 
+def finder_leg(log, frames=500, size=512, n=1000, reps=10, cpu_frames=20, seed=20261213):
+    """
+    Local-maxima peak finding (daoFindLocalMaxima) over 500 frames of 512 x 512
+    resident on the GPU, searched as 500 planes with z_range 0 (each frame on
+    its own, as MaximaFinder.findMaxima does per frame): 1000 Gaussian spots per
+    frame (sigma 1.5, heights U(5, 50)) on unit noise, margin 11, radius 3,
+    threshold 6, values rounded to 0.1 (ties).  The taken counters are reset
+    before each call.  Bytes per pixel: the image (8 B) and taken (4 B) read,
+    the flag mask written and read back (2 B).
+    """
+    import ctypes
+    import torch
+    from storm_analysis_amd.sa_library import dao_finder
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import finder_oracle as FD
+    rng = numpy.random.default_rng(seed)
+    yy, xx = numpy.mgrid[0:size, 0:size]
+    base = numpy.zeros((size, size))
+    for y, x, h in zip(rng.uniform(0, size, n), rng.uniform(0, size, n), rng.uniform(5, 50, n)):
+        y0, y1, x0, x1 = int(max(0, y - 6)), int(min(size, y + 7)), int(max(0, x - 6)), int(min(size, x + 7))
+        base[y0:y1, x0:x1] += h * numpy.exp(-((yy[y0:y1, x0:x1] - y) ** 2 + (xx[y0:y1, x0:x1] - x) ** 2) / 4.5)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    st = torch.round((torch.from_numpy(base).cuda()[None] +
+                      torch.randn((frames, size, size), dtype=torch.float64, device="cuda", generator=g)) * 10) / 10
+    st = st.contiguous()
+    taken = torch.zeros(st.shape, dtype=torch.int32, device="cuda")
+    cap = frames * size * size // 8
+    out = torch.empty((4, cap), dtype=torch.float64, device="cuda")
+    zv = numpy.arange(frames, dtype=numpy.float64)
+    lib = dao_finder.loadFinderC()
+    nf = ctypes.c_int(0)
+    ptrs = [ctypes.c_void_p(out[i].data_ptr()) for i in range(4)]
+
+    def call():
+        taken.zero_()
+        torch.cuda.synchronize()
+        rc = lib.daoFindLocalMaxima(ctypes.c_void_p(st.data_ptr()), ctypes.c_void_p(taken.data_ptr()), frames, size,
+                                    size, 11, 0, 3.0, 6.0, zv.ctypes.data_as(ctypes.c_void_p), cap, *ptrs,
+                                    ctypes.byref(nf))
+        if rc != 0:
+            raise RuntimeError("daoFindLocalMaxima failed (%d)" % rc)
+        return nf.value
+    found = call()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    dt = (time.perf_counter() - t0) / reps
+    host = st[:cpu_frames].cpu().numpy()
+    res = out[:, :found].cpu().numpy()
+    ref_so = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "libia_utilities.so")
+    exact = True
+    t1 = time.perf_counter()
+    if os.path.exists(ref_so):
+        rlib = ctypes.cdll.LoadLibrary(ref_so)
+        kind, what = "reference", "the reference's findLocalMaxima (ia_utilities.c built from source, oracle/_ref)"
+        outs = []
+        for f in range(cpu_frames):
+            tk = [numpy.zeros((size, size), dtype=numpy.int32)]
+            (z, y, x, h), _ = FD.flm_call(rlib, [host[f]], tk, 11, 0, 3.0, 6.0, [float(f)], size * size)
+            outs.append((z, y, x, h))
+    else:
+        kind, what = "port", "the numpy restatement (oracle/finder_oracle.py)"
+        outs = []
+        for f in range(cpu_frames):
+            tk = [numpy.zeros((size, size), dtype=numpy.int32)]
+            outs.append(FD.find_local_maxima([host[f]], tk, 11, 0, 3.0, 6.0, [float(f)], size * size))
+    dt_cpu = (time.perf_counter() - t1) / cpu_frames
+    want = numpy.concatenate([numpy.stack(o) for o in outs], axis=1)
+    exact = bool(numpy.array_equal(res[:, :want.shape[1]], want))
+    px = frames * size * size
+    bytes_ = px * 14
+    gbs = bytes_ / dt / 1e9
+    log("finder: %d frames %dx%d, %d maxima: %.3f ms per call (%.0f GB/s, %.1f%% of HBM), first %d frames "
+        "identical to the %s: %s; CPU (1 thread) %.2f ms per frame" %
+        (frames, size, size, found, dt * 1e3, gbs, 100 * gbs / HBM_PEAK_GBS, cpu_frames, kind, exact, dt_cpu * 1e3))
+    return {"frames": frames, "frame_size": size, "spots_per_frame": n, "maxima": found, "ms_per_call": dt * 1e3,
+            "frames_per_s": frames / dt, "bytes_per_call": bytes_,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS},
+            "identical_to_cpu": exact,
+            "cpu_baseline": {"value": 1.0 / dt_cpu, "unit": "frames/s", "cores": 1, "kind": kind,
+                             "sample": "%d frames through %s" % (cpu_frames, what)}}
+
+
 def dense_leg(log, seed=20261210):
     """
     High-density overlap stress: one 2048^2 frame with 20000 emitters
@@ -693,6 +778,7 @@ def main():
 
     ingest = None if args.no_ingest else ingest_leg(log)
     filters = None if args.no_filter else filter_leg(log)
+    finder = None if args.no_finder else finder_leg(log)
 
     hm = meas[head]
     line = {
@@ -728,6 +814,7 @@ def main():
         "serving": serving,
         "ingest": ingest,
         "filters": filters,
+        "finder": finder,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }
