@@ -76,6 +76,7 @@ def parse():
     p.add_argument("--no-ingest", action="store_true", help="skip the frame-ingest leg")
     p.add_argument("--no-filter", action="store_true", help="skip the post-fit filter leg")
     p.add_argument("--no-finder", action="store_true", help="skip the local-maxima finder leg")
+    p.add_argument("--no-mfilter", action="store_true", help="skip the FFT convolution leg")
     p.add_argument("--waves", type=int, default=0,
                    help="waves per workgroup of the batched fitter (0: the kernel's default)")
     p.add_argument("--serve-seconds", type=float, default=3.0,
@@ -597,6 +598,52 @@ def finder_leg(log, frames=500, size=512, n=1000, reps=10, cpu_frames=20, seed=2
                              "sample": "%d frames through %s" % (cpu_frames, what)}}
 
 
+def mfilter_leg(log, frames=500, size=512, reps=10, cpu_frames=20, seed=20261214):
+    """
+    FFT convolution (libmatched_filter.so, daoMatchedFilterConvolveBatch) of
+    500 frames of 512 x 512 resident on the GPU with a unit-sum Gaussian PSF
+    (sigma 8, the background filter's scale): one batched forward transform,
+    the spectrum product and one batched inverse transform.  Bytes per pixel
+    at the boundary: the frame read and the result written (16 B).
+    """
+    import torch
+    from storm_analysis_amd.sa_library import dao_mfilter
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle"))
+    import mfilter_oracle as MO
+    psf = MO.gaussian_psf((size, size), 8.0)
+    flt = dao_mfilter.MatchedFilter(psf)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    images = torch.rand((frames, size, size), dtype=torch.float64, device="cuda", generator=g) * 100.0
+    out = flt.convolveBatch(images)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = flt.convolveBatch(images)
+    dt = (time.perf_counter() - t0) / reps
+    host = images[:cpu_frames].cpu().numpy()
+    got = out[:cpu_frames].cpu().numpy()
+    orc = MO.MatchedFilterOracle(psf)
+    t1 = time.perf_counter()
+    want = [orc.convolve(host[f]) for f in range(cpu_frames)]
+    dt_cpu = (time.perf_counter() - t1) / cpu_frames
+    err = max(float(numpy.max(numpy.abs(got[f] - want[f])) / numpy.max(numpy.abs(want[f])))
+              for f in range(cpu_frames))
+    flt.cleanup()
+    bytes_ = frames * size * size * 16
+    gbs = bytes_ / dt / 1e9
+    log("mfilter: %d frames %dx%d: %.3f ms per batch (%.0f GB/s at the boundary, %.1f%% of HBM), max relative "
+        "error vs the oracle %.1e; CPU (numpy FFT, 1 thread) %.2f ms per frame" %
+        (frames, size, size, dt * 1e3, gbs, 100 * gbs / HBM_PEAK_GBS, err, dt_cpu * 1e3))
+    return {"frames": frames, "frame_size": size, "psf_sigma": 8.0, "ms_per_batch": dt * 1e3,
+            "frames_per_s": frames / dt, "bytes_per_batch": bytes_,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS},
+            "max_rel_error_vs_oracle": err,
+            "cpu_baseline": {"value": 1.0 / dt_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
+                             "sample": "%d frames through the numpy FFT restatement (oracle/mfilter_oracle.py); "
+                                       "the reference's C needs FFTW, not in its sources" % cpu_frames}}
+
+
 def dense_leg(log, seed=20261210):
     """
     High-density overlap stress: one 2048^2 frame with 20000 emitters
@@ -779,6 +826,7 @@ def main():
     ingest = None if args.no_ingest else ingest_leg(log)
     filters = None if args.no_filter else filter_leg(log)
     finder = None if args.no_finder else finder_leg(log)
+    mfilter = None if args.no_mfilter else mfilter_leg(log)
 
     hm = meas[head]
     line = {
@@ -815,6 +863,7 @@ def main():
         "ingest": ingest,
         "filters": filters,
         "finder": finder,
+        "mfilter": mfilter,
         "converged_per_step": hm["converged_per_step"],
         "generate_s": t_gen,
     }
