@@ -70,3 +70,34 @@ def gaussian_psf(shape, sigma, center=None):
     cx, cy = (shape[0] / 2, shape[1] / 2) if center is None else center
     g = numpy.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2.0 * sigma * sigma))
     return g / g.sum()
+
+
+def finder_prepare(image, background, fit, camera_variance=None):
+    """bg_var and foreground before the matched filters (fitting.py:452-478)."""
+    bg_var = background + fit
+    if camera_variance is not None:
+        bg_var += camera_variance
+    foreground = image - background - fit
+    return bg_var, foreground
+
+
+def finder_snr(foreground, bg_var, peak_mask=None):
+    """The signal-to-noise image after the matched filters (fitting.py:466-494)."""
+    bg_var = bg_var.copy()
+    mask = (bg_var <= 0.1)
+    if numpy.sum(mask) > 0:
+        bg_var[mask] = 0.1
+    snr = foreground / numpy.sqrt(bg_var)
+    if peak_mask is not None:
+        snr = snr * peak_mask
+    return snr
+
+
+def peak_finder_snr(image, background, fit, fg_filter=None, var_filter=None, peak_mask=None, camera_variance=None):
+    """PeakFinderGaussian.peakFinder up to the maxima search (fitting.py:444-498)."""
+    bg_var, foreground = finder_prepare(image, background, fit, camera_variance)
+    if var_filter is not None:
+        bg_var = var_filter.convolve(bg_var)
+    if fg_filter is not None:
+        foreground = fg_filter.convolve(foreground)
+    return finder_snr(foreground, bg_var, peak_mask)
