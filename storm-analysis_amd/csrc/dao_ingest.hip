@@ -12,7 +12,8 @@
 //     pixel repeated: rows first, then columns, which for a separable mirror is
 //     the same as mirroring each axis).
 // Here one pass over HBM does all three for a batch of frames: each output
-// pixel is computed from its mirrored source pixel.  The
+// pixel is computed from its mirrored source pixel, a workgroup per four
+// padded rows.  The
 // arithmetic is the reference's, operation for operation (one subtraction,
 // the gain * rqe product, one multiplication, the clamp), so the results are
 // bit-identical.  Bytes per output pixel: 2 (raw, shared by up to 4 padded
@@ -34,34 +35,40 @@ __device__ inline int mirror(int o, int n, int pad) {
   return i;
 }
 
+constexpr int kRowsPerBlock = 4;
+
 // mode 0: raw uint16 counts -> photo-electrons; mode 1: float64 frames,
-// padded only.  Workgroup (row, frame) writes one padded output row; its
-// threads stride along it, so the stores are contiguous and no thread divides.
+// padded only.  Workgroup (row block, frame) writes kRowsPerBlock padded
+// output rows; its threads stride along each, so the stores are contiguous
+// and no thread divides.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_ingest(const void* in, int W, int H, int big_endian, const double* off_map,
                                                 const double* gain_map, const double* rqe_map, double off,
                                                 double gain, double rqe, int pad, double* out) {
-  const int Wo = W + 2 * pad;
-  const int oy = blockIdx.x;
+  const int Wo = W + 2 * pad, Ho = H + 2 * pad;
   const long long f = blockIdx.y;
-  const int sy = mirror(oy, H, pad);
-  double* orow = out + (f * (H + 2 * pad) + oy) * (long long)Wo;
-  const long long srow = (long long)sy * W;
-  for (int ox = threadIdx.x; ox < Wo; ox += blockDim.x) {
-    const long long s = srow + mirror(ox, W, pad);
-    double v;
-    if (MODE == 0) {
-      uint16_t a = static_cast<const uint16_t*>(in)[f * W * H + s];
-      if (big_endian) a = (uint16_t)((a >> 8) | (a << 8));
-      const double o = off_map ? off_map[s] : off;
-      const double g = gain_map ? gain_map[s] : gain;
-      const double q = rqe_map ? rqe_map[s] : rqe;
-      v = ((double)a - o) * (g * q);
-      if (v < 1.0) v = 1.0;
-    } else {
-      v = static_cast<const double*>(in)[f * W * H + s];
+  for (int r = 0; r < kRowsPerBlock; ++r) {
+    const int oy = blockIdx.x * kRowsPerBlock + r;
+    if (oy >= Ho) break;
+    const int sy = mirror(oy, H, pad);
+    double* orow = out + (f * Ho + oy) * (long long)Wo;
+    const long long srow = (long long)sy * W;
+    for (int ox = threadIdx.x; ox < Wo; ox += blockDim.x) {
+      const long long s = srow + mirror(ox, W, pad);
+      double v;
+      if (MODE == 0) {
+        uint16_t a = static_cast<const uint16_t*>(in)[f * W * H + s];
+        if (big_endian) a = (uint16_t)((a >> 8) | (a << 8));
+        const double o = off_map ? off_map[s] : off;
+        const double g = gain_map ? gain_map[s] : gain;
+        const double q = rqe_map ? rqe_map[s] : rqe;
+        v = ((double)a - o) * (g * q);
+        if (v < 1.0) v = 1.0;
+      } else {
+        v = static_cast<const double*>(in)[f * W * H + s];
+      }
+      orow[ox] = v;
     }
-    orow[ox] = v;
   }
 }
 
@@ -86,8 +93,10 @@ bool sizes_ok(int n, int W, int H, int pad) {
          (long long)H + 2LL * pad <= 0x7fffffffLL;
 }
 
-// One workgroup per padded output row of every frame.
-dim3 grid_for(int H, int n, int pad) { return dim3((unsigned)(H + 2 * pad), (unsigned)n); }
+// One workgroup per kRowsPerBlock padded output rows of every frame.
+dim3 grid_for(int H, int n, int pad) {
+  return dim3((unsigned)((H + 2 * pad + kRowsPerBlock - 1) / kRowsPerBlock), (unsigned)n);
+}
 
 }  // namespace
 
