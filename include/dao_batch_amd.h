@@ -14,6 +14,14 @@
  * "finder"/"testing", n x 7 for "text"/"hdf5").  Pointers flagged
  * `on_device` are HIP device pointers (for example torch CUDA tensors).
  * Calls are synchronous, one call per batch at a time.
+ *
+ * Errors.  No entry point aborts the process.  Functions returning int
+ * return 0 (or the count they describe) on success and a negative code on
+ * failure: -1 argument out of range, -2 HIP runtime failure, -3 more peaks
+ * than a frame's capacity (checked before any work: nothing changes), -4
+ * executor made no progress, -5 some frames failed (daoBatchFailedFrames;
+ * the other frames' results stand), -6 other.  daoGetLastError
+ * (dao_fit_amd.h) returns the calling thread's last code and message.
  */
 #ifndef DAO_BATCH_AMD_H
 #define DAO_BATCH_AMD_H
@@ -60,7 +68,7 @@ int daoBatchLoadFrame(daoBatch *b, int frame, int width, int height, const doubl
                       const double *background, const double *rqe, const double *scmos_cal, const double *rows,
                       int n_rows, const char *p_type, int on_device);
 /* Empty slot `frame`; daoBatchFit skips empty slots. */
-void daoBatchClearFrame(daoBatch *b, int frame);
+int daoBatchClearFrame(daoBatch *b, int frame);
 /* daoBatchLoadFrame for n slots at once (uniform calibration: rqe = 1,
  * scmos = 0), with one synchronisation: frames[k] receives the
  * widths[k] x heights[k] image and background and n_rows[k] start peaks.
@@ -69,7 +77,7 @@ int daoBatchLoadFrames(daoBatch *b, int n, const int *frames, const int *widths,
                        const double *const *images, const double *const *backgrounds, const double *const *rows,
                        const int *n_rows, const char *p_type, int on_device);
 /* Empty n slots at once. */
-void daoBatchClearFrames(daoBatch *b, int n, const int *frames);
+int daoBatchClearFrames(daoBatch *b, int n, const int *frames);
 
 /* Create a batch of n_frames frames; peak_caps[f] bounds the peaks of frame f.
  * Returns NULL without a GPU or with unsupported sizes. */
@@ -78,34 +86,52 @@ daoBatch *daoBatchCreate(const daoBatchModel *model, int n_frames, const int *wi
 void daoBatchDestroy(daoBatch *b);
 
 /* Per-frame camera calibration (host arrays; NULL = rqe 1 / scmos 0). */
-void daoBatchSetCalibration(daoBatch *b, int frame, const double *rqe, const double *scmos_cal);
+int daoBatchSetCalibration(daoBatch *b, int frame, const double *rqe, const double *scmos_cal);
 
 /* mFitNewImage / mFitNewBackground (multi_fit.c:1360-1408) for every frame. */
-void daoBatchNewImages(daoBatch *b, const double *images, int on_device);
-void daoBatchNewBackgrounds(daoBatch *b, const double *backgrounds, int on_device);
+int daoBatchNewImages(daoBatch *b, const double *images, int on_device);
+int daoBatchNewBackgrounds(daoBatch *b, const double *backgrounds, int on_device);
 /* mFitAnscombeTransformImage for every frame (ALS models). */
-void daoBatchAnscombe(daoBatch *b);
+int daoBatchAnscombe(daoBatch *b);
 
-/* daoNewPeaks (dao_fit.c:1582-1766) for every frame: counts[f] rows for
- * frame f, rows concatenated in frame order. */
-void daoBatchNewPeaks(daoBatch *b, const double *rows, const int *counts, const char *p_type, int on_device);
+/* daoNewPeaks (dao_fit.c:1582-1766) for every frame: counts[f] (n_frames
+ * entries) rows for frame f, rows concatenated in frame order.  -3, with
+ * nothing changed, if a frame's held peaks plus its new ones exceed its
+ * capacity. */
+int daoBatchNewPeaks(daoBatch *b, const double *rows, const int *counts, const char *p_type, int on_device);
 
-/* doFit for every frame; returns the number of sweeps run. */
+/* doFit for every frame; returns the number of sweeps run, or -5 when a
+ * frame's scheduler wait exceeded its bound: that frame is emptied (its
+ * state mid-fit is not the sequential one) and flagged, the other frames
+ * are fitted as usual and can be read. */
 int daoBatchFit(daoBatch *b, int max_iterations);
+/* Frames that failed in the last daoBatchFit: flags[f] = 1 (n_frames
+ * entries); returns their number. */
+int daoBatchFailedFrames(daoBatch *b, int32_t *flags);
+/* Polls after which a scheduler wait of the fitter gives up and fails its
+ * frame (0: the default, far beyond any wait of a correct schedule). */
+void daoBatchSetWaitLimit(daoBatch *b, long long polls);
 
 /* Per-frame read-back.  out: nfit x 9 doubles per peak
  * {x, y, z, height, background, xsigma, ysigma, error, iterations}. */
+int daoBatchNFrames(daoBatch *b);
 int daoBatchNFit(daoBatch *b, int frame);
-void daoBatchGetPeaks(daoBatch *b, int frame, double *out, int32_t *status);
-/* All frames at once, concatenated in frame order (sum of nfit rows). */
-void daoBatchGetAllPeaks(daoBatch *b, double *out, int32_t *status);
+int daoBatchGetPeaks(daoBatch *b, int frame, double *out, int32_t *status);
+/* All frames at once, concatenated in frame order (sum of nfit rows): into
+ * host arrays, or into device arrays (no host copy). */
+int daoBatchGetAllPeaks(daoBatch *b, double *out, int32_t *status);
+int daoBatchGetAllPeaksDevice(daoBatch *b, double *out, int32_t *status);
 /* Counters of the frame since the last call, fitData order:
  * n_dposv, n_iterations, n_lost, n_margin, n_neg_fi, n_neg_height, n_non_converged, n_non_decr. */
 void daoBatchCounters(daoBatch *b, int frame, int32_t *out8);
-/* Fit image (f / rqe) of one frame. */
-void daoBatchGetFitImage(daoBatch *b, int frame, double *out);
+/* Fit image (f / rqe) of one frame (mFitGetFitImage, multi_fit.c:708-720). */
+int daoBatchGetFitImage(daoBatch *b, int frame, double *out);
+/* Fit images (residual = 0: f / rqe) or residuals (1: x - f,
+ * mFitGetResidual multi_fit.c:880-888) of every frame, concatenated in frame
+ * order at the frames' current sizes, into host or device memory. */
+int daoBatchGetFitImages(daoBatch *b, double *out, int residual, int on_device);
 /* mFitRemoveErrorPeaks for every frame. */
-void daoBatchRemoveErrorPeaks(daoBatch *b);
+int daoBatchRemoveErrorPeaks(daoBatch *b);
 
 /* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
  * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */

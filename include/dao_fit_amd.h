@@ -6,10 +6,13 @@
  * storm_analysis/sa_library/{multi_fit.c,dao_fit.c}, with the argument types
  * the reference's ctypes binding declares (storm_analysis/sa_library/
  * dao_fit_c.py:147-238).  Inputs are copied by the callee; outputs go to
- * caller-allocated arrays; there are no return codes (per-peak failures are
- * status == ERROR plus the counters in the handle).  Calls are synchronous:
- * each returns when its GPU work has finished.  Handles share no state and
- * may be used from different threads, one call per handle at a time.
+ * caller-allocated arrays; as in the reference there are no return codes
+ * (per-peak failures are status == ERROR plus the counters in the handle).
+ * A failure of the call itself (a HIP runtime error, the scheduler giving
+ * up) never aborts the process: the call returns and daoGetLastError()
+ * reports it.  Calls are synchronous: each returns when its GPU work has
+ * finished.  Handles share no state and may be used from different
+ * threads, one call per handle at a time.
  *
  * The batch / serving entry points (daoBatch*) have no counterpart in the
  * reference; they are declared in dao_batch_amd.h.
@@ -149,6 +152,15 @@ void mFitSetPeakStatus(fitData *fd, int32_t *status);
  * instead of one (0, the default: one workgroup).  Same results; shorter
  * doFit / mFitIterateLM latency for frames with many peaks. */
 void daoSetTeam(fitData *fd, int peaks_per_workgroup);
+/* No counterpart in the reference: the code of the calling thread's last
+ * failed call of this library (0: none; the codes of dao_batch_amd.h) and,
+ * if msg is not NULL, its message (at most len bytes with the terminating
+ * zero).  Clears the error. */
+int daoGetLastError(char *msg, int len);
+/* No counterpart in the reference: polls after which a scheduler wait of
+ * the fitter gives up (0: the default).  A wait that gives up fails the
+ * call (mFitIterateLM / mFitDoFit) and drops the frame's peaks. */
+void daoSetWaitLimit(fitData *fd, long long polls);
 
 #ifdef __cplusplus
 }

@@ -42,7 +42,8 @@ void cleanup(filter *flt);
 
 /* n images (device, n x x_size x y_size, n <= 65535) convolved into result
  * (device) in one batched transform; 0 on success, -1 for bad arguments, -2 for a device
- * error.  No counterpart in the reference. */
+ * error.  The images are left untouched (the transform runs on a copy in
+ * result; result may also be images itself).  No counterpart in the reference. */
 int daoMatchedFilterConvolveBatch(filter *flt, const double *images, int n, double *result);
 
 #ifdef __cplusplus

@@ -198,7 +198,8 @@ struct FitArgs {
   int* frame_sweeps;    // [F]: sweeps run by this launch
   const int* frame_list;
   int max_more;         // doFit's max_iterations; 0 = exactly one sweep
-  int* err_flag;        // set when a wait exceeds its (very large) bound
+  int* frame_err;       // [F]: set when one of the frame's waits exceeds its bound
+  long long spin_limit; // polls before a scheduler wait gives up
   // frame-per-team fitter only
   TeamCtl* team_ctl;            // [F]
   int* team_cur;                // [seats]: whether the seat's wave is inside a run

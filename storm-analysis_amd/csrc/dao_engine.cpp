@@ -10,16 +10,38 @@
 namespace dao {
 
 void fatal(const char* file, int line, const char* what, const char* msg) {
-  std::fprintf(stderr, "dao_fit (HIP): %s:%d: %s failed: %s\n", file, line, what, msg);
-  std::fflush(stderr);
-  std::abort();
+  char buf[512];
+  const char* base = std::strrchr(file, '/');
+  std::snprintf(buf, sizeof(buf), "%s:%d: %s failed: %s", base ? base + 1 : file, line, what, msg);
+  throw Error(kErrHip, buf);
 }
 
-static void fail(const char* msg) {
-  std::fprintf(stderr, "dao_fit (HIP): %s\n", msg);
-  std::fflush(stderr);
-  std::abort();
+void fail(int code, const char* msg) { throw Error(code, msg); }
+
+namespace {
+thread_local int t_code = kOk;
+thread_local std::string t_msg;
+}  // namespace
+
+void set_last_error(int code, const char* msg) {
+  t_code = code;
+  t_msg = msg ? msg : "";
 }
+
+int take_last_error(char* msg, int len) {
+  const int c = t_code;
+  if (msg != nullptr && len > 0) {
+    std::snprintf(msg, (size_t)len, "%s", t_msg.c_str());
+  }
+  t_code = kOk;
+  t_msg.clear();
+  return c;
+}
+
+// Polls before a scheduler wait of the fitters gives up: far longer than
+// any wait of a correct schedule (a wait lasts at most one LM turn of an
+// earlier overlapping op, or one transition).
+constexpr long long kSpinLimit = 1ll << 22;
 
 void roi_table(int roi, std::vector<int>& rx, std::vector<int>& ry) {
   rx.clear();
@@ -45,15 +67,28 @@ static T* dalloc(long long n) {
   return (T*)p;
 }
 
+// Frees never throw: they also run in destructors, and a failed free (a
+// device in an error state) leaves nothing to recover.
 template <typename T>
 static void dfree(T*& p) {
-  if (p) DAO_CHECK(hipFree((void*)p));
+  if (p) (void)hipFree((void*)p);
   p = nullptr;
 }
 
+// A failure part-way through construction releases what was allocated and
+// rethrows.
 Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
+  try {
+    init(specs);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void Engine::init(const std::vector<FrameSpec>& specs) {
   int dev_count = 0;
-  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) fail("no HIP device available");
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) fail(kErrHip, "no HIP device available");
   int dev = 0;
   DAO_CHECK(hipGetDevice(&dev));
   DAO_CHECK(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -63,7 +98,7 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
 
   roi_table(m.roi, roi_x, roi_y);
   m.nroi = (int)roi_x.size();
-  if (m.roi < 1 || m.roi > 32) fail("roi_size must be in [1, 32]");
+  if (m.roi < 1 || m.roi > 32) fail(kErrArg, "roi_size must be in [1, 32]");
   std::vector<int> packed(m.nroi);
   for (int j = 0; j < m.nroi; j++) packed[j] = roi_x[j] | (roi_y[j] << 16);
   d_roi = dalloc<int>(m.nroi);
@@ -123,7 +158,9 @@ Engine::Engine(const Model& mm, const std::vector<FrameSpec>& specs) : m(mm) {
   DAO_CHECK(hipStreamSynchronize(st));
 }
 
-Engine::~Engine() {
+Engine::~Engine() { release(); }
+
+void Engine::release() {
   if (st) (void)hipStreamSynchronize(st);
   dfree(pk.rec);
   dfree(pk.bk);
@@ -237,7 +274,7 @@ void Engine::grow_total_cap(long long c) {
 
 void Engine::ensure_frame_cap(int f, int need) {
   if (F != 1) {
-    if (need > fr[f].cap) fail("frame peak capacity exceeded");
+    if (need > fr[f].cap) fail(kErrCapacity, "more peaks than the frame's capacity");
     return;
   }
   if (need <= fr[0].cap && need <= cap) return;
@@ -369,7 +406,7 @@ void Engine::run_ops(int kind, const double* rows, int fmt, int first_slot) {
     }
     int rem = h[0];
     if (rem == 0) break;
-    if (rem >= last) fail("ordered executor made no progress");
+    if (rem >= last) fail(kErrSched, "the ordered executor made no progress");
     last = rem;
     group = std::min(group * 2, 32);
   }
@@ -431,13 +468,15 @@ int Engine::fit_frames(int max_more) {
   if (d_sweeps == nullptr) {
     d_sweeps = dalloc<int>(F);
     d_flist = dalloc<int>(F);
-    d_err = dalloc<int>(1);
-    launch_fill_i32(d_err, 1, 0, st);
+    d_err = dalloc<int>(F);
   }
+  launch_fill_i32(d_err, F, 0, st);
+  failed.assign(F, 0);
+  failed_count = 0;
   std::vector<int> tab;
   int nseats = 0;
   const int nblocks = team_blocks(tab, nseats);
-  int* h = pin(sizeof(int) * ((size_t)F + 1 + tab.size()));
+  int* h = pin(sizeof(int) * (2 * (size_t)F + tab.size()));
   std::memcpy(h, list.data(), sizeof(int) * list.size());
   DAO_CHECK(hipMemcpyAsync(d_flist, h, sizeof(int) * list.size(), hipMemcpyHostToDevice, st));
   launch_fill_i32(d_sweeps, F, 0, st);
@@ -453,7 +492,7 @@ int Engine::fit_frames(int max_more) {
       team_tab_cap = std::max((int)tab.size(), 2 * team_tab_cap);
       d_team_tab = dalloc<int>(team_tab_cap);
     }
-    int* ht = h + F + 1;
+    int* ht = h + 2 * F;
     std::memcpy(ht, tab.data(), sizeof(int) * tab.size());
     DAO_CHECK(hipMemcpyAsync(d_team_tab, ht, sizeof(int) * tab.size(), hipMemcpyHostToDevice, st));
     launch_team_init(d_team_ctl, F, d_team_cur, nseats, st);
@@ -477,7 +516,8 @@ int Engine::fit_frames(int max_more) {
   a.frame_sweeps = d_sweeps;
   a.frame_list = d_flist;
   a.max_more = max_more;
-  a.err_flag = d_err;
+  a.frame_err = d_err;
+  a.spin_limit = spin_limit > 0 ? spin_limit : kSpinLimit;
   a.team_ctl = d_team_ctl;
   a.team_cur = d_team_cur;
   a.team_block_frame = nblocks > 0 ? d_team_tab : nullptr;
@@ -496,11 +536,21 @@ int Engine::fit_frames(int max_more) {
   DAO_CHECK(hipGetLastError());
   if (timing) DAO_CHECK(hipEventRecord(ev1, st));
   DAO_CHECK(hipMemcpyAsync(h, d_sweeps, sizeof(int) * F, hipMemcpyDeviceToHost, st));
-  DAO_CHECK(hipMemcpyAsync(h + F, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  DAO_CHECK(hipMemcpyAsync(h + F, d_err, sizeof(int) * F, hipMemcpyDeviceToHost, st));
   sync();
-  if (h[F] != 0) fail("a fitter wait exceeded its bound (scheduler error)");
   int most = 0;
   for (int f = 0; f < F; f++) most = std::max(most, h[f]);
+  // A frame whose wait gave up stopped part-way: its fit state is not the
+  // sequential one, so it is emptied and reported; the others stand.
+  bool any = false;
+  for (int f = 0; f < F; f++) {
+    if (h[F + f] == 0) continue;
+    failed[f] = 1;
+    failed_count++;
+    fr[f].nfit = 0;
+    any = true;
+  }
+  if (any) upload_frames();
   sweeps_total += most;
   if (timing) {
     float ms = 0.0f;
@@ -554,7 +604,7 @@ int Engine::sweep() {
     launch_collect_escapes(sc, pk, d_override, st);
     attempt++;
     max_attempts_seen = std::max(max_attempts_seen, attempt);
-    if (attempt > 64) fail("sweep did not settle after 64 attempts");
+    if (attempt > 64) fail(kErrSched, "a sweep did not settle after 64 attempts");
     sc.nops = select(SEL_RUNNING_BAD);
   }
   // mFitRecenterPeaks (multi_fit.c:1644-1690)
@@ -610,7 +660,7 @@ void Engine::new_peaks_multi(const std::vector<int>& frames, const std::vector<c
   for (size_t k : order) {
     int f = frames[k], n = counts[k];
     if (n == 0) continue;
-    if (fr[f].nfit + n > fr[f].cap) fail("frame peak capacity exceeded");
+    if (fr[f].nfit + n > fr[f].cap) fail(kErrCapacity, "more peaks than the frame's capacity");
     DAO_CHECK(hipMemcpyAsync(d_rows + at * width, rows_dev[k], sizeof(double) * (size_t)n * width,
                              hipMemcpyDeviceToDevice, st));
     launch_iota(ops + at, n, fr[f].peak0 + fr[f].nfit, st);
@@ -765,7 +815,7 @@ long long Engine::total_nfit() const {
   return n;
 }
 
-void Engine::pack_all(double* out, int32_t* status) {
+void Engine::pack_all(double* out, int32_t* status, bool on_device) {
   long long total = total_nfit();
   if (total == 0) return;
   std::vector<int> off(F);
@@ -779,10 +829,37 @@ void Engine::pack_all(double* out, int32_t* status) {
   double* d = stage(rows_bytes + stat_bytes + sizeof(int) * F + 64);
   int* d_status = (int*)((char*)d + rows_bytes);
   int* d_off = d_status + total;
-  DAO_CHECK(hipMemcpyAsync(d_off, off.data(), sizeof(int) * F, hipMemcpyHostToDevice, st));
-  launch_pack_peaks(pk, d_fr, d_off, m, cap, d, d_status, st);
-  DAO_CHECK(hipMemcpyAsync(out, d, rows_bytes, hipMemcpyDeviceToHost, st));
-  DAO_CHECK(hipMemcpyAsync(status, d_status, stat_bytes, hipMemcpyDeviceToHost, st));
+  int* h_off = pin(sizeof(int) * F);
+  std::memcpy(h_off, off.data(), sizeof(int) * F);
+  DAO_CHECK(hipMemcpyAsync(d_off, h_off, sizeof(int) * F, hipMemcpyHostToDevice, st));
+  if (on_device) {
+    // straight into the caller's device arrays
+    launch_pack_peaks(pk, d_fr, d_off, m, cap, out, status, st);
+  } else {
+    launch_pack_peaks(pk, d_fr, d_off, m, cap, d, d_status, st);
+    DAO_CHECK(hipMemcpyAsync(out, d, rows_bytes, hipMemcpyDeviceToHost, st));
+    DAO_CHECK(hipMemcpyAsync(status, d_status, stat_bytes, hipMemcpyDeviceToHost, st));
+  }
+  sync();
+}
+
+// Fit images (f / rqe, or the residual x - f) of every frame, concatenated
+// in frame order at the frames' current sizes.
+void Engine::get_images_all(double* out, int residual, bool on_device) {
+  const long long n = frame_pixels_total();
+  if (n == 0) return;
+  double* d = on_device ? out : stage(sizeof(double) * n);
+  if (packed()) {
+    launch_fit_image(px, 0, npix, d, residual, st);
+  } else {
+    long long at = 0;
+    for (int f = 0; f < F; f++) {
+      const long long nf = (long long)fr[f].W * fr[f].H;
+      launch_fit_image(px, fr[f].pix0, nf, d + at, residual, st);
+      at += nf;
+    }
+  }
+  if (!on_device) DAO_CHECK(hipMemcpyAsync(out, d, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   sync();
 }
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "dao_device.h"
@@ -17,7 +19,55 @@
 
 namespace dao {
 
+// Error codes of the library's entry points (0: success).  No entry point
+// aborts the process: a failure inside the engine throws dao::Error, the
+// C-ABI function that was called catches it, records it as the calling
+// thread's last error (daoGetLastError) and returns (batch entry points
+// return the code).
+enum ErrCode {
+  kOk = 0,
+  kErrArg = -1,       // an argument is out of range
+  kErrHip = -2,       // a HIP runtime call failed (device memory, launch, ...)
+  kErrCapacity = -3,  // more peaks than a frame's slots
+  kErrSched = -4,     // the ordered executor stopped making progress
+  kErrFrame = -5,     // some frames of a batch failed (daoBatchFailedFrames)
+  kErrInternal = -6   // anything else
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+// Throw dao::Error(kErrHip) for a failed HIP call.
 [[noreturn]] void fatal(const char* file, int line, const char* what, const char* msg);
+// Throw dao::Error(code, msg).
+[[noreturn]] void fail(int code, const char* msg);
+// The calling thread's last error.
+void set_last_error(int code, const char* msg);
+int take_last_error(char* msg, int len);
+
+// Run fn; an exception becomes the calling thread's last error and its code
+// is returned (0 when fn returns normally).
+template <class Fn>
+int guarded(Fn&& fn) {
+  try {
+    fn();
+    return kOk;
+  } catch (const Error& e) {
+    set_last_error(e.code, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error(kErrInternal, "host memory exhausted");
+    return kErrInternal;
+  } catch (const std::exception& e) {
+    set_last_error(kErrInternal, e.what());
+    return kErrInternal;
+  } catch (...) {
+    set_last_error(kErrInternal, "unknown failure");
+    return kErrInternal;
+  }
+}
 
 // kernel launchers (dao_kernels.hip)
 
@@ -125,8 +175,13 @@ class Engine {
   int fit(int max_iterations);
   int running_count();
   // sweeps of every frame with peaks, one persistent workgroup per frame:
-  // max_more = 0 runs exactly one sweep (mFitIterateLM), otherwise doFit
+  // max_more = 0 runs exactly one sweep (mFitIterateLM), otherwise doFit.
+  // A frame whose scheduler wait gave up is left empty (nfit = 0) and
+  // flagged in `failed`; the other frames' results stand.
   int fit_frames(int max_more);
+  std::vector<int> failed;   // per frame: failed in the last fit
+  int failed_count = 0;
+  long long spin_limit = 0;  // polls before a scheduler wait gives up (0: kSpinLimit)
   bool use_rounds = false;  // multi-kernel executor instead of fit_frames
   // timing of the LM kernel (HIP events on the engine's stream)
   bool timing = false;
@@ -143,8 +198,10 @@ class Engine {
   void get_sums(int f, int what, std::vector<double>& out);
   void get_jacobian(int f, std::vector<double>& out);
   void get_image(int f, double* out_host, int residual);
-  // all frames' live peaks as rows of 9 doubles + status (host buffers)
-  void pack_all(double* out_host, int32_t* status_host);
+  // all frames' live peaks as rows of 9 doubles + status (host or device
+  // buffers)
+  void pack_all(double* out, int32_t* status, bool on_device);
+  void get_images_all(double* out, int residual, bool on_device);
   long long total_nfit() const;
   // every frame at once from device arrays concatenated at the frames'
   // current sizes
@@ -162,6 +219,8 @@ class Engine {
   void sync();
 
  private:
+  void init(const std::vector<FrameSpec>& specs);
+  void release();  // frees everything (also after a failed construction)
   void alloc_sched(long long c);
   void free_sched();
   void grow_total_cap(long long c);

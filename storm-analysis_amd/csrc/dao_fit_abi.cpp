@@ -52,6 +52,36 @@ void set_model(Handle* h, int shape, int errm, int jac) {
   h->eng->set_model(h->model);
 }
 
+
+// The frame fitter leaves a frame whose scheduler wait gave up empty; for
+// the single-frame handle that is the call's failure.
+void check_frame(Handle* h) {
+  if (h->eng->failed_count > 0) {
+    h->pub.nfit = 0;
+    dao::fail(dao::kErrFrame, "a scheduler wait of the fitter exceeded its bound; the frame's peaks were dropped");
+  }
+}
+
+// Entry points run their body through dao::guarded: a failure becomes the
+// calling thread's last error (daoGetLastError) and the call returns.
+template <class Fn>
+void run(Fn&& fn) {
+  (void)dao::guarded(fn);
+}
+template <class T, class Fn>
+T run_value(T on_error, Fn&& fn) {
+  T v = on_error;
+  if (dao::guarded([&] { v = fn(); }) != dao::kOk) return on_error;
+  return v;
+}
+
+int peak_format(const char* p_type) {
+  if (!std::strcmp(p_type, "finder")) return dao::PF_FINDER;
+  if (!std::strcmp(p_type, "testing")) return dao::PF_TESTING;
+  if (!std::strcmp(p_type, "text") || !std::strcmp(p_type, "hdf5")) return dao::PF_TEXT;
+  return -1;
+}
+
 }  // namespace
 
 extern "C" {
@@ -60,11 +90,13 @@ fitData* daoInitialize(double* rqe, double* scmos_cal, double tol, int im_size_x
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     std::fprintf(stderr, "dao_fit (HIP): no GPU device is available\n");
+    dao::set_last_error(dao::kErrHip, "no GPU device is available");
     return nullptr;
   }
   if (roi_size < 1 || roi_size > 32 || im_size_x < 1 || im_size_y < 1) {
     std::fprintf(stderr, "dao_fit (HIP): unsupported sizes (roi %d, image %d x %d)\n", roi_size, im_size_x,
                  im_size_y);
+    dao::set_last_error(dao::kErrArg, "unsupported sizes (roi_size must be 1..32)");
     return nullptr;
   }
   Handle* h = new Handle();
@@ -101,8 +133,18 @@ fitData* daoInitialize(double* rqe, double* scmos_cal, double tol, int im_size_x
   spec[0].W = im_size_x;
   spec[0].H = im_size_y;
   spec[0].cap = 0;
-  h->eng.reset(new Engine(m, spec));
-  h->eng->set_calibration(0, rqe, scmos_cal);
+  const int rc = dao::guarded([&] {
+    h->eng.reset(new Engine(m, spec));
+    h->eng->set_calibration(0, rqe, scmos_cal);
+  });
+  if (rc != dao::kOk) {
+    char msg[256];
+    const int code = dao::take_last_error(msg, sizeof(msg));
+    std::fprintf(stderr, "dao_fit (HIP): daoInitialize: %s\n", msg);
+    dao::set_last_error(code, msg);
+    delete h;
+    return nullptr;
+  }
   return &h->pub;
 }
 
@@ -145,41 +187,34 @@ void daoInitializeZ(fitData* fd, double* wx, double* wy, double z_min, double z_
 }
 
 void daoNewPeaks(fitData* fd, double* peak_params, char* p_type, int n_peaks) {
-  Handle* h = H(fd);
-  fitData& p = h->pub;
-  int fmt;
-  if (!std::strcmp(p_type, "finder")) {
-    fmt = dao::PF_FINDER;
-  } else if (!std::strcmp(p_type, "testing")) {
-    fmt = dao::PF_TESTING;
-  } else if (!std::strcmp(p_type, "text") || !std::strcmp(p_type, "hdf5")) {
-    fmt = dao::PF_TEXT;
-  } else {
-    fmt = -1;
-  }
-  // storage policy of mFitNewPeaks (multi_fit.c:1428-1477): growing drops
-  // ERROR peaks and rounds the capacity to blocks of 500
-  if (p.nfit + n_peaks > p.max_nfit) {
-    int n_alloc = kPeakBlock * ((p.nfit + n_peaks) / kPeakBlock + 1);
-    std::vector<int32_t> st, keep;
-    h->eng->get_status(0, st);
-    for (int i = 0; i < (int)st.size(); i++)
-      if (st[i] != dao::ST_ERROR) keep.push_back(i);
-    h->eng->compact(0, keep);
-    p.max_nfit = n_alloc;
-    p.nfit = (int)keep.size();
-  }
-  if (fmt < 0) {
-    // the reference allocates and initialises the slots, then leaves them
-    // untouched (dao_fit.c:1753-1755)
-    std::printf("Unrecognized peak type '%s'!\n", p_type);
-    return;
-  }
-  if (n_peaks > 0) {
-    h->eng->new_peaks(0, peak_params, fmt, n_peaks);
-    p.nfit += n_peaks;
-  }
-  add_counters(h);
+  run([&] {
+    Handle* h = H(fd);
+    fitData& p = h->pub;
+    const int fmt = peak_format(p_type);
+    // storage policy of mFitNewPeaks (multi_fit.c:1428-1477): growing drops
+    // ERROR peaks and rounds the capacity to blocks of 500
+    if (p.nfit + n_peaks > p.max_nfit) {
+      int n_alloc = kPeakBlock * ((p.nfit + n_peaks) / kPeakBlock + 1);
+      std::vector<int32_t> st, keep;
+      h->eng->get_status(0, st);
+      for (int i = 0; i < (int)st.size(); i++)
+        if (st[i] != dao::ST_ERROR) keep.push_back(i);
+      h->eng->compact(0, keep);
+      p.max_nfit = n_alloc;
+      p.nfit = (int)keep.size();
+    }
+    if (fmt < 0) {
+      // the reference allocates and initialises the slots, then leaves them
+      // untouched (dao_fit.c:1753-1755)
+      std::printf("Unrecognized peak type '%s'!\n", p_type);
+      return;
+    }
+    if (n_peaks > 0) {
+      h->eng->new_peaks(0, peak_params, fmt, n_peaks);
+      p.nfit += n_peaks;
+    }
+    add_counters(h);
+  });
 }
 
 void daoCleanup(fitData* fd) {
@@ -190,141 +225,173 @@ void daoCleanup(fitData* fd) {
 }
 
 void mFitNewImage(fitData* fd, double* image) {
-  Handle* h = H(fd);
-  h->eng->new_image(0, image);
-  h->pub.nfit = 0;
+  run([&] {
+    Handle* h = H(fd);
+    h->pub.nfit = 0;
+    h->eng->new_image(0, image);
+  });
 }
 
-void mFitNewBackground(fitData* fd, double* background) { H(fd)->eng->new_background(0, background); }
+void mFitNewBackground(fitData* fd, double* background) {
+  run([&] { H(fd)->eng->new_background(0, background); });
+}
 
-void mFitAnscombeTransformImage(fitData* fd) { H(fd)->eng->anscombe(0); }
+void mFitAnscombeTransformImage(fitData* fd) {
+  run([&] { H(fd)->eng->anscombe(0); });
+}
 
 void mFitIterateLM(fitData* fd) {
-  Handle* h = H(fd);
-  if (h->eng->use_rounds)
-    h->eng->sweep();
-  else
-    h->eng->fit_frames(0);
-  add_counters(h);
+  run([&] {
+    Handle* h = H(fd);
+    if (h->eng->use_rounds)
+      h->eng->sweep();
+    else
+      h->eng->fit_frames(0);
+    add_counters(h);
+    check_frame(h);
+  });
 }
 
 // doFit in one call: one sweep, then more while any peak is RUNNING, at most
 // max_iterations more (dao_fit_c.py:332-356); the same result as calling
-// mFitIterateLM / mFitGetUnconverged in that loop.  Returns the sweeps run.
+// mFitIterateLM / mFitGetUnconverged in that loop.  Returns the sweeps run
+// (-1 on failure).
 int mFitDoFit(fitData* fd, int max_iterations) {
-  Handle* h = H(fd);
-  int s;
-  if (h->eng->use_rounds) {
-    s = h->eng->fit(max_iterations);
-  } else {
-    s = h->eng->fit_frames(max_iterations);
-  }
-  add_counters(h);
-  return s;
+  return run_value(-1, [&] {
+    Handle* h = H(fd);
+    const int s = h->eng->use_rounds ? h->eng->fit(max_iterations) : h->eng->fit_frames(max_iterations);
+    add_counters(h);
+    check_frame(h);
+    return s;
+  });
 }
 
-int mFitGetUnconverged(fitData* fd) { return H(fd)->eng->count_status(0, dao::ST_RUNNING); }
+int mFitGetUnconverged(fitData* fd) {
+  return run_value(-1, [&] { return H(fd)->eng->count_status(0, dao::ST_RUNNING); });
+}
 
-int mFitGetNError(fitData* fd) { return H(fd)->eng->count_status(0, dao::ST_ERROR); }
+int mFitGetNError(fitData* fd) {
+  return run_value(-1, [&] { return H(fd)->eng->count_status(0, dao::ST_ERROR); });
+}
 
-void mFitGetFitImage(fitData* fd, double* out) { H(fd)->eng->get_image(0, out, 0); }
+void mFitGetFitImage(fitData* fd, double* out) {
+  run([&] { H(fd)->eng->get_image(0, out, 0); });
+}
 
-void mFitGetResidual(fitData* fd, double* out) { H(fd)->eng->get_image(0, out, 1); }
+void mFitGetResidual(fitData* fd, double* out) {
+  run([&] { H(fd)->eng->get_image(0, out, 1); });
+}
 
 void mFitGetPeakPropertyDouble(fitData* fd, double* values, char* what) {
-  Handle* h = H(fd);
-  Engine& e = *h->eng;
-  const fitData& p = h->pub;
-  std::vector<double> v;
-  int n = p.nfit;
-  auto put = [&](double off) {
-    for (int i = 0; i < n; i++) values[i] = v[i] + off;
-  };
-  if (!std::strcmp(what, "background")) {
-    e.get_param(0, dao::P_BG, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "bg_sum")) {
-    e.get_sums(0, 0, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "error")) {
-    e.get_error(0, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "fg_sum") || !std::strcmp(what, "fg_sum_sc")) {
-    e.get_sums(0, 1, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "height")) {
-    e.get_param(0, dao::P_HEIGHT, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "jacobian")) {
-    e.get_jacobian(0, v);
-    for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
-  } else if (!std::strcmp(what, "sum")) {
-    e.get_sums(0, 2, v);
-    put(0.0);
-  } else if (!std::strcmp(what, "x")) {
-    e.get_param(0, dao::P_XC, v);
-    put(p.xoff);
-  } else if (!std::strcmp(what, "xsigma")) {
-    e.get_param(0, dao::P_XW, v);
-    for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
-  } else if (!std::strcmp(what, "y")) {
-    e.get_param(0, dao::P_YC, v);
-    put(p.yoff);
-  } else if (!std::strcmp(what, "ysigma")) {
-    e.get_param(0, dao::P_YW, v);
-    for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
-  } else if (!std::strcmp(what, "z")) {
-    e.get_param(0, dao::P_ZC, v);
-    put(p.zoff);
-  } else {
-    std::printf("Unrecognized parameter '%s'!\n", what);
-  }
+  run([&] {
+    Handle* h = H(fd);
+    Engine& e = *h->eng;
+    const fitData& p = h->pub;
+    std::vector<double> v;
+    int n = p.nfit;
+    auto put = [&](double off) {
+      for (int i = 0; i < n; i++) values[i] = v[i] + off;
+    };
+    if (!std::strcmp(what, "background")) {
+      e.get_param(0, dao::P_BG, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "bg_sum")) {
+      e.get_sums(0, 0, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "error")) {
+      e.get_error(0, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "fg_sum") || !std::strcmp(what, "fg_sum_sc")) {
+      e.get_sums(0, 1, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "height")) {
+      e.get_param(0, dao::P_HEIGHT, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "jacobian")) {
+      e.get_jacobian(0, v);
+      for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
+    } else if (!std::strcmp(what, "sum")) {
+      e.get_sums(0, 2, v);
+      put(0.0);
+    } else if (!std::strcmp(what, "x")) {
+      e.get_param(0, dao::P_XC, v);
+      put(p.xoff);
+    } else if (!std::strcmp(what, "xsigma")) {
+      e.get_param(0, dao::P_XW, v);
+      for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
+    } else if (!std::strcmp(what, "y")) {
+      e.get_param(0, dao::P_YC, v);
+      put(p.yoff);
+    } else if (!std::strcmp(what, "ysigma")) {
+      e.get_param(0, dao::P_YW, v);
+      for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
+    } else if (!std::strcmp(what, "z")) {
+      e.get_param(0, dao::P_ZC, v);
+      put(p.zoff);
+    } else {
+      std::printf("Unrecognized parameter '%s'!\n", what);
+    }
+  });
 }
 
 void mFitGetPeakPropertyInt(fitData* fd, int32_t* values, char* what) {
-  Handle* h = H(fd);
-  std::vector<int32_t> v;
-  if (!std::strcmp(what, "iterations")) {
-    h->eng->get_iterations(0, v);
-  } else if (!std::strcmp(what, "status")) {
-    h->eng->get_status(0, v);
-  } else {
-    std::printf("Unrecognized parameter '%s'!\n", what);
-    return;
-  }
-  for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
+  run([&] {
+    Handle* h = H(fd);
+    std::vector<int32_t> v;
+    if (!std::strcmp(what, "iterations")) {
+      h->eng->get_iterations(0, v);
+    } else if (!std::strcmp(what, "status")) {
+      h->eng->get_status(0, v);
+    } else {
+      std::printf("Unrecognized parameter '%s'!\n", what);
+      return;
+    }
+    for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
+  });
 }
 
 void mFitRemoveErrorPeaks(fitData* fd) {
-  Handle* h = H(fd);
-  std::vector<int32_t> st, keep;
-  h->eng->get_status(0, st);
-  for (int i = 0; i < (int)st.size(); i++)
-    if (st[i] != dao::ST_ERROR) keep.push_back(i);
-  h->eng->compact(0, keep);
-  h->pub.nfit = (int)keep.size();
+  run([&] {
+    Handle* h = H(fd);
+    std::vector<int32_t> st, keep;
+    h->eng->get_status(0, st);
+    for (int i = 0; i < (int)st.size(); i++)
+      if (st[i] != dao::ST_ERROR) keep.push_back(i);
+    h->eng->compact(0, keep);
+    h->pub.nfit = (int)keep.size();
+  });
 }
 
 void mFitRemoveRunningPeaks(fitData* fd) {
-  Handle* h = H(fd);
-  std::vector<int32_t> st, keep;
-  h->eng->get_status(0, st);
-  for (int i = 0; i < (int)st.size(); i++) {
-    if (st[i] != dao::ST_RUNNING)
-      keep.push_back(i);
-    else
-      h->pub.n_non_converged++;
-  }
-  h->eng->compact(0, keep);
-  h->pub.nfit = (int)keep.size();
+  run([&] {
+    Handle* h = H(fd);
+    std::vector<int32_t> st, keep;
+    h->eng->get_status(0, st);
+    int dropped = 0;
+    for (int i = 0; i < (int)st.size(); i++) {
+      if (st[i] != dao::ST_RUNNING)
+        keep.push_back(i);
+      else
+        dropped++;
+    }
+    h->eng->compact(0, keep);
+    h->pub.n_non_converged += dropped;
+    h->pub.nfit = (int)keep.size();
+  });
 }
 
-void mFitSetPeakStatus(fitData* fd, int32_t* status) { H(fd)->eng->set_status(0, status); }
+void mFitSetPeakStatus(fitData* fd, int32_t* status) {
+  run([&] { H(fd)->eng->set_status(0, status); });
+}
 
 void daoSetTeam(fitData* fd, int peaks_per_workgroup) {
   Handle* h = H(fd);
   h->model.team = peaks_per_workgroup < 0 ? 0 : peaks_per_workgroup;
   h->eng->set_model(h->model);
 }
+
+int daoGetLastError(char* msg, int len) { return dao::take_last_error(msg, len); }
+
+void daoSetWaitLimit(fitData* fd, long long polls) { H(fd)->eng->spin_limit = polls > 0 ? polls : 0; }
 
 }  // extern "C"

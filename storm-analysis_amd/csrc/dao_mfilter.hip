@@ -72,8 +72,11 @@ bool ok(hipError_t e) { return e == hipSuccess; }
 bool ok(hipfftResult r) { return r == HIPFFT_SUCCESS; }
 
 // spec = FFT(in); spec *= psf_fft; out = inverse FFT(spec).
-bool run(filter* f, hipfftHandle fwd, hipfftHandle bwd, const double* in, double* spec, double* out, int n) {
-  if (!ok(hipfftExecD2Z(fwd, const_cast<double*>(in), reinterpret_cast<hipfftDoubleComplex*>(spec)))) return false;
+// `in` is the transform's input and may be overwritten by it (rocFFT's
+// out-of-place transforms are allowed to use their input as work space):
+// callers pass a buffer of their own, never the caller's images.
+bool run(filter* f, hipfftHandle fwd, hipfftHandle bwd, double* in, double* spec, double* out, int n) {
+  if (!ok(hipfftExecD2Z(fwd, in, reinterpret_cast<hipfftDoubleComplex*>(spec)))) return false;
   k_cmul<<<dim3(grid_for(f->fft_size), (unsigned)n), kThreads, 0, f->st>>>(
       reinterpret_cast<double2*>(spec), reinterpret_cast<const double2*>(f->psf_fft), f->fft_size);
   if (!ok(hipGetLastError())) return false;
@@ -187,7 +190,12 @@ int daoMatchedFilterConvolveBatch(filter* f, const double* images, int n, double
       return -2;
     f->batch_n = n;
   }
-  if (!run(f, f->fwd_n, f->bwd_n, images, f->spec_n, result, n)) return -2;
+  // the forward transform may overwrite its input: it reads a copy of the
+  // images in `result`, which the inverse transform then fills
+  if (result != images &&
+      !ok(hipMemcpyAsync(result, images, sizeof(double) * f->image_size * (size_t)n, hipMemcpyDeviceToDevice, f->st)))
+    return -2;
+  if (!run(f, f->fwd_n, f->bwd_n, result, f->spec_n, result, n)) return -2;
   return ok(hipStreamSynchronize(f->st)) ? 0 : -2;
 }
 

@@ -1575,6 +1575,7 @@ struct FpwShared {
   int4 curbox[NW];  // its box
   int wsum[NW];
   int cnt[C_COUNT];
+  int failed;       // a wait of this frame gave up: the frame stops
 };
 
 // Order-preserving list of the frame's peaks that satisfy `mode`
@@ -1642,9 +1643,11 @@ __device__ inline double* op_journal(const FitArgs& a, const Frame& fr, int slot
 }
 
 // Wait until op j may start.  Returns false when an earlier op escaped (op j
-// must not run now) or the wait bound was hit.
+// must not run now) or the wait bound was hit; then frame f is marked failed
+// (frame_err[f] for the host, sh.failed for the workgroup) and every later
+// wait of the frame gives up at once.
 template <int NW>
-__device__ bool fpw_wait(const FitArgs& a, int j, int4 b, FpwShared<NW>& sh) {
+__device__ bool fpw_wait(const FitArgs& a, int f, int j, int4 b, FpwShared<NW>& sh) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (long long spin = 0;; spin++) {
     bool blocked = false;
@@ -1661,8 +1664,11 @@ __device__ bool fpw_wait(const FitArgs& a, int j, int4 b, FpwShared<NW>& sh) {
       return true;
     }
     if (__hip_atomic_load(&sh.esc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < j) return false;
-    if (spin > (1ll << 22)) {
-      if (lane == 0) atomicExch(a.err_flag, 1);
+    if (spin > a.spin_limit || __hip_atomic_load(&sh.failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+      if (lane == 0) {
+        __hip_atomic_store(&sh.failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        atomicExch(&a.frame_err[f], 1);
+      }
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -1679,7 +1685,7 @@ __device__ inline void fpw_set_cur(int* cur, int v) {
 // first, the turn gets the whole frame as its box (it then runs alone and
 // cannot escape again), and the run resumes from it.
 template <int PPL, int SHAPE, int NW, bool MP>
-__device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, const int* list, FpwShared<NW>& sh,
+__device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int n, const int* list, FpwShared<NW>& sh,
                         double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
@@ -1706,7 +1712,7 @@ __device__ void fpw_run(const FitArgs& a, const Frame& fr, int kind, int n, cons
       const int4 b = uniform_i4(a.box[slot]);
       if (lane == 0) sh.curbox[wv] = b;
       fpw_set_cur(&sh.cur[wv], j);
-      if (!fpw_wait<NW>(a, j, b, sh)) {
+      if (!fpw_wait<NW>(a, f, j, b, sh)) {
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
@@ -1791,10 +1797,14 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   const Frame fr = a.frames[f];
   const Model& m = a.m;
   if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.failed = 0;
   __syncthreads();
   int sweeps = 0;
   int n = 0;  // RUNNING peaks at the sweep's start: a.run_list[peak0 .. peak0 + n)
   for (int it = 0;; it++) {
+    // a wait gave up (every wave passed a barrier since): the frame's state
+    // is not the sequential one, so it stops and the host reports it
+    if (sh.failed) break;
     n = it == 0 ? fpw_list(a, fr, SEL_RUNNING, sh.wsum, nullptr, 0, a.run_list)
                 : fpw_list(a, fr, SEL_RUNNING, sh.wsum, a.run_list, n, a.run_list);
     if (it > 0 && (n == 0 || it > a.max_more)) break;
@@ -1814,7 +1824,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
       a.pk.bk[p] = r;
     }
     __syncthreads();
-    fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_LM, n, a.run_list, sh, slice);
+    fpw_run<PPL, SHAPE, NW, MP>(a, f, fr, OP_LM, n, a.run_list, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
     const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum, a.run_list, n, a.op_peak);
     if (nr > 0) {
@@ -1831,7 +1841,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
         a.box[slot] = b;
       }
       __syncthreads();
-      fpw_run<PPL, SHAPE, NW, MP>(a, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
+      fpw_run<PPL, SHAPE, NW, MP>(a, f, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
     }
   }
   __syncthreads();
@@ -1867,7 +1877,6 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
 // co-resident), and the smallest unfinished op can always run, so every
 // run ends.
 
-constexpr long long kTeamSpin = 1ll << 22;  // polls before a wait gives up (seconds)
 constexpr int kActive = 1;                  // seat of a wave inside team_ops (else kIdle)
 
 struct TeamShared {
@@ -1888,11 +1897,21 @@ constexpr int kActRun = 0, kActLead = 1, kActStop = 2;
 // they also leave the hot loop (take, wait, turn) fewer registers to spill:
 // a team turn got ~12% faster than with the whole op loop out of line.
 
+// Every wait of a team polls at most a.spin_limit times.  A wait that gives
+// up marks frame f failed (frame_err[f]); every wait of the frame's team
+// then gives up at its next poll and every workgroup of the team stops, so
+// the launch ends and the host reports the frame.
+__device__ inline bool team_failed(const FitArgs& a, int f) { return uniform_i(MemDev::ld(&a.frame_err[f])) != 0; }
+__device__ inline void team_fail(const FitArgs& a, int f) {
+  if (lane_id() == 0) MemDev::st(&a.frame_err[f], 1);
+}
+
 // Wave 0: wait for a generation other than g, or claim the transition out
 // of g once it is due (g == 0: the frame's first run; then: the run's ops
 // are all done, or one escaped).  Returns the action; sh.gen its generation.
-__device__ __noinline__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, TeamShared& sh) {
+__device__ __noinline__ int team_await(const FitArgs& a, int f, TeamCtl* ctl, int g, TeamShared& sh) {
   for (long long spin = 0;; spin++) {
+    if (team_failed(a, f)) return kActStop;
     const int ph = uniform_i(MemDev::ld(&ctl->phase));
     if (ph != g) {
       dev_order();
@@ -1915,8 +1934,8 @@ __device__ __noinline__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, Te
         return kActLead;
       }
     }
-    if (spin > kTeamSpin) {
-      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+    if (spin > a.spin_limit) {
+      team_fail(a, f);
       return kActStop;
     }
     __builtin_amdgcn_s_sleep(2);
@@ -1924,17 +1943,18 @@ __device__ __noinline__ int team_await(const FitArgs& a, TeamCtl* ctl, int g, Te
 }
 
 // Wave 0 of the claimant: wait until no seat of the frame holds or is
-// taking an op.
-__device__ __noinline__ bool team_quiet(const FitArgs& a, int s0, int s1) {
+// taking an op.  False if the frame failed.
+__device__ __noinline__ bool team_quiet(const FitArgs& a, int f, int s0, int s1) {
   for (long long spin = 0;; spin++) {
+    if (team_failed(a, f)) return false;
     bool busy = false;
     for (int s = s0 + lane_id(); s < s1; s += kWave) busy |= MemDev::ld(&a.team_cur[s]) != kIdle;
     if (!__any(busy)) {
       dev_order();
       return true;
     }
-    if (spin > kTeamSpin) {
-      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+    if (spin > a.spin_limit) {
+      team_fail(a, f);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -1947,7 +1967,7 @@ __device__ __noinline__ bool team_quiet(const FitArgs& a, int s0, int s1) {
 // held or waiting ones.  `lo` is this wave's bound below which every op has
 // finished; it only grows during a run.  False when an earlier op escaped
 // (op j must not run now) or the wait gave up.
-__device__ bool team_wait(const FitArgs& a, TeamCtl* ctl, const Frame& fr, int j, int4 b, int& lo) {
+__device__ bool team_wait(const FitArgs& a, int f, TeamCtl* ctl, const Frame& fr, int j, int4 b, int& lo) {
   for (long long spin = 0;; spin++) {
     while (lo < j) {
       const int i = lo + lane_id();
@@ -1969,8 +1989,8 @@ __device__ bool team_wait(const FitArgs& a, TeamCtl* ctl, const Frame& fr, int j
       return true;
     }
     if (uniform_i(MemDev::ld(&ctl->esc)) < j) return false;
-    if (spin > kTeamSpin) {
-      if (lane_id() == 0) atomicExch(a.err_flag, 1);
+    if (spin > a.spin_limit || team_failed(a, f)) {
+      team_fail(a, f);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -1980,7 +2000,7 @@ __device__ bool team_wait(const FitArgs& a, TeamCtl* ctl, const Frame& fr, int j
 // One wave takes and runs ops of generation g until none is left, a
 // transition is claimed or an op escapes.  Its seat reads kActive meanwhile.
 template <int PPL, int SHAPE, bool MP>
-__device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
+__device__ void team_ops(const FitArgs& a, int f, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
                          TeamShared& sh, double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id();
@@ -1990,13 +2010,14 @@ __device__ void team_ops(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int g,
   for (;;) {
     if (uniform_i(MemDev::ld(&ctl->lead)) != g) break;
     if (uniform_i(MemDev::ld(&ctl->esc)) != kNoEscape) break;
+    if (team_failed(a, f)) break;
     int j = 0;
     if (lane == 0) j = MemDev::add(&ctl->next, 1);
     j = uniform_i(j);
     if (j >= n) break;
     const int slot = fr.peak0 + j;
     const int4 b = uniform_i4(MemDev::ld(&a.box[slot]));
-    if (!team_wait(a, ctl, fr, j, b, lo)) break;
+    if (!team_wait(a, f, ctl, fr, j, b, lo)) break;
     const int p = uniform_i(MemDev::ld(&(kind == OP_LM ? a.run_list : a.op_peak)[slot]));
     int d[C_COUNT];
 #pragma unroll
@@ -2185,7 +2206,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
   int g = 0;  // the last generation this workgroup ran (0: none)
   for (;;) {
     if (threadIdx.x < kWave) {
-      const int act = team_await(a, ctl, g, sh);
+      const int act = team_await(a, f, ctl, g, sh);
       if (threadIdx.x == 0) {
         sh.act = act;
         if (act == kActRun) {
@@ -2199,14 +2220,18 @@ __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
     __syncthreads();
     if (act == kActStop) break;
     if (act == kActLead) {
-      if (threadIdx.x < kWave) team_quiet(a, s0, s1);
+      if (threadIdx.x < kWave) {
+        const bool quiet = team_quiet(a, f, s0, s1);
+        if (threadIdx.x == 0) sh.act = quiet ? kActLead : kActStop;
+      }
       __syncthreads();
+      if (sh.act == kActStop) break;
       team_transition<PPL>(a, fr, ctl, f, gen, sh, slice);
       continue;  // g unchanged: the next await sees the new generation
     }
     g = gen;
     if (kind == kTeamDone) break;
-    team_ops<PPL, SHAPE, MP>(a, fr, ctl, g, kind, n, seat, sh, slice);
+    team_ops<PPL, SHAPE, MP>(a, f, fr, ctl, g, kind, n, seat, sh, slice);
     __syncthreads();
   }
   __syncthreads();

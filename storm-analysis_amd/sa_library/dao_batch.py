@@ -14,7 +14,7 @@ import numpy
 from numpy.ctypeslib import ndpointer
 
 from . import loadclib
-from .dao_fit_c import MultiFitterException, formatPeaksGaussianPSF
+from .dao_fit_c import MultiFitterException, formatPeaksGaussianPSF, raiseOnLibraryError
 
 SHAPES = {"2dfixed": 0, "2d": 1, "3d": 2, "Z": 3}
 ERROR_MODELS = {"MLE": 0, "ALS": 1, "LS": 2, "DWLS": 3, "FWLS": 4}
@@ -39,6 +39,35 @@ class daoBatchModel(ctypes.Structure):
                 ("precision", ctypes.c_int)]
 
 
+class FrameFitError(MultiFitterException):
+    """
+    Some frames of a batch failed in doFit (a scheduler wait of the fitter
+    exceeded its bound).  `frames` lists them; they were emptied.  The other
+    frames were fitted and their results can be read as usual.
+    """
+    def __init__(self, message, frames):
+        super(FrameFitError, self).__init__(message)
+        self.frames = list(frames)
+
+
+_FRAME_FAILED = -5
+_BATCH_ENTRY_POINTS = (
+    "daoBatchCreate", "daoBatchSetCalibration", "daoBatchNewImages", "daoBatchNewBackgrounds",
+    "daoBatchAnscombe", "daoBatchNewPeaks", "daoBatchFit", "daoBatchGetPeaks", "daoBatchGetAllPeaks",
+    "daoBatchGetAllPeaksDevice", "daoBatchGetFitImage", "daoBatchGetFitImages", "daoBatchRemoveErrorPeaks",
+    "daoBatchLoadFrame", "daoBatchClearFrame", "daoBatchLoadFrames", "daoBatchClearFrames")
+
+
+def _batch_error(code, msg, func, args):
+    if code == _FRAME_FAILED and func.__name__ == "daoBatchFit":
+        lib = loadBatchC()
+        n = lib.daoBatchNFrames(args[0])
+        flags = numpy.zeros(max(n, 1), dtype=numpy.int32)
+        lib.daoBatchFailedFrames(args[0], flags)
+        return FrameFitError("daoBatchFit: " + msg, numpy.flatnonzero(flags[:n]).tolist())
+    return MultiFitterException("%s failed (code %d): %s" % (func.__name__, code, msg))
+
+
 _lib = None
 
 
@@ -52,6 +81,8 @@ def loadBatchC():
         lib.daoBatchCreate.argtypes = [ctypes.POINTER(daoBatchModel), ctypes.c_int, i32p, i32p, i32p]
         lib.daoBatchCreate.restype = vp
         lib.daoBatchDestroy.argtypes = [vp]
+        lib.daoBatchNFrames.argtypes = [vp]
+        lib.daoBatchNFrames.restype = ctypes.c_int
         lib.daoBatchSetCalibration.argtypes = [vp, ctypes.c_int, vp, vp]
         lib.daoBatchNewImages.argtypes = [vp, vp, ctypes.c_int]
         lib.daoBatchNewBackgrounds.argtypes = [vp, vp, ctypes.c_int]
@@ -59,12 +90,17 @@ def loadBatchC():
         lib.daoBatchNewPeaks.argtypes = [vp, vp, i32p, ctypes.c_char_p, ctypes.c_int]
         lib.daoBatchFit.argtypes = [vp, ctypes.c_int]
         lib.daoBatchFit.restype = ctypes.c_int
+        lib.daoBatchFailedFrames.argtypes = [vp, i32p]
+        lib.daoBatchFailedFrames.restype = ctypes.c_int
+        lib.daoBatchSetWaitLimit.argtypes = [vp, ctypes.c_longlong]
         lib.daoBatchNFit.argtypes = [vp, ctypes.c_int]
         lib.daoBatchNFit.restype = ctypes.c_int
         lib.daoBatchGetPeaks.argtypes = [vp, ctypes.c_int, f64p, i32p]
         lib.daoBatchGetAllPeaks.argtypes = [vp, f64p, i32p]
+        lib.daoBatchGetAllPeaksDevice.argtypes = [vp, vp, vp]
         lib.daoBatchCounters.argtypes = [vp, ctypes.c_int, i32p]
         lib.daoBatchGetFitImage.argtypes = [vp, ctypes.c_int, f64p]
+        lib.daoBatchGetFitImages.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
         lib.daoBatchRemoveErrorPeaks.argtypes = [vp]
         lib.daoBatchLoadFrame.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
                                           ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
@@ -77,7 +113,7 @@ def loadBatchC():
         lib.daoBatchSetTiming.argtypes = [vp, ctypes.c_int]
         lib.daoBatchSetTuning.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         lib.daoBatchStats.argtypes = [vp, f64p]
-        _lib = lib
+        _lib = raiseOnLibraryError(lib, _BATCH_ENTRY_POINTS, _batch_error)
     return _lib
 
 
@@ -202,7 +238,10 @@ class BatchFitter(object):
             if counts is None:
                 raise MultiFitterException("counts are required with concatenated rows")
         counts = numpy.ascontiguousarray(counts, dtype=numpy.int32)
-        if numpy.any(counts > self.caps):
+        if counts.shape != (self.n_frames,):
+            raise MultiFitterException("counts must have one entry per frame (%d)" % self.n_frames)
+        held = numpy.array([self.getNFit(f) for f in range(self.n_frames)], dtype=numpy.int64)
+        if numpy.any(counts < 0) or numpy.any(held + counts > self.caps):
             raise MultiFitterException("more peaks than the frame's capacity")
         p, dev, keep = _ptr(rows, numpy.float64)
         self.clib.daoBatchNewPeaks(self.handle, p, counts, ctypes.c_char_p(peaks_type.encode()), dev)
@@ -289,7 +328,16 @@ class BatchFitter(object):
             self.clib.daoBatchClearFrames(self.handle, len(slots), numpy.asarray(slots, dtype=numpy.int32))
 
     def doFit(self, max_iterations=2000):
+        """
+        MultiFitter.doFit for every frame; returns the sweeps run.  Raises
+        FrameFitError (naming the frames) if some frames failed; the other
+        frames' results stand.
+        """
         return self.clib.daoBatchFit(self.handle, max_iterations)
+
+    def setWaitLimit(self, polls):
+        """Polls after which a scheduler wait gives up and fails its frame (0: default)."""
+        self.clib.daoBatchSetWaitLimit(self.handle, int(polls))
 
     def removeErrorPeaks(self):
         self.clib.daoBatchRemoveErrorPeaks(self.handle)
@@ -327,6 +375,36 @@ class BatchFitter(object):
         out = numpy.zeros(self.shapes[frame])
         self.clib.daoBatchGetFitImage(self.handle, frame, out)
         return out
+
+    def getFitImages(self, residual=False, device=None):
+        """
+        Fit images (f / rqe) -- or residuals x - f -- of every frame,
+        concatenated in frame order (flat).  device: None for a numpy array,
+        else a torch device: the images are written there without a host copy.
+        """
+        n = int(sum(h * w for h, w in self.shapes))
+        if device is None:
+            out = numpy.zeros(n)
+            self.clib.daoBatchGetFitImages(self.handle, out.ctypes.data_as(ctypes.c_void_p), int(residual), 0)
+            return out
+        import torch
+        out = torch.empty(n, dtype=torch.float64, device=device)
+        torch.cuda.current_stream(out.device).synchronize()
+        self.clib.daoBatchGetFitImages(self.handle, ctypes.c_void_p(out.data_ptr()), int(residual), 1)
+        return out
+
+    def getAllPeaksDevice(self, device):
+        """getAllPeaks into torch tensors on `device` (rows n x 9, status n, counts)."""
+        import torch
+        counts = numpy.array([self.getNFit(f) for f in range(self.n_frames)], dtype=numpy.int64)
+        n = int(counts.sum())
+        out = torch.empty((n, 9), dtype=torch.float64, device=device)
+        status = torch.empty(n, dtype=torch.int32, device=device)
+        if n:
+            torch.cuda.current_stream(out.device).synchronize()
+            self.clib.daoBatchGetAllPeaksDevice(self.handle, ctypes.c_void_p(out.data_ptr()),
+                                                ctypes.c_void_p(status.data_ptr()))
+        return out, status, counts
 
     def setTuning(self, waves=0, margin=-1, team=-1):
         """

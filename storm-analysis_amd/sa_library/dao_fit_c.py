@@ -143,6 +143,49 @@ def setDaoFitArgtypes(lib):
     lib.daoNewPeaks.argtypes = [vp, f64, ctypes.c_char_p, ctypes.c_int]
     if hasattr(lib, "daoSetTeam"):
         lib.daoSetTeam.argtypes = [vp, ctypes.c_int]
+    if hasattr(lib, "daoSetWaitLimit"):
+        lib.daoSetWaitLimit.argtypes = [vp, ctypes.c_longlong]
+    return raiseOnLibraryError(lib, _FIT_ENTRY_POINTS)
+
+
+_FIT_ENTRY_POINTS = (
+    "mFitAnscombeTransformImage", "mFitGetFitImage", "mFitGetNError", "mFitGetPeakPropertyDouble",
+    "mFitGetPeakPropertyInt", "mFitGetResidual", "mFitGetUnconverged", "mFitIterateLM", "mFitDoFit",
+    "mFitNewBackground", "mFitNewImage", "mFitRemoveErrorPeaks", "mFitRemoveRunningPeaks", "mFitSetPeakStatus",
+    "daoInitialize", "daoNewPeaks")
+
+
+def lastLibraryError(lib):
+    """(code, message) of the calling thread's last failed call of the library; clears it."""
+    msg = ctypes.create_string_buffer(512)
+    code = lib.daoGetLastError(msg, len(msg))
+    return code, msg.value.decode(errors="replace")
+
+
+def raiseOnLibraryError(lib, names, exception=None):
+    """
+    Not in the reference: the library never aborts the process; a call
+    that fails returns and leaves its error for daoGetLastError().  This
+    makes each named entry point raise (MultiFitterException, or what
+    exception(code, message, func, args) returns) after such a call.  A
+    library without daoGetLastError (the oracle) is returned unchanged.
+    """
+    if not hasattr(lib, "daoGetLastError"):
+        return lib
+    lib.daoGetLastError.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    lib.daoGetLastError.restype = ctypes.c_int
+
+    def errcheck(result, func, args):
+        code, msg = lastLibraryError(lib)
+        if code != 0:
+            if exception is not None:
+                raise exception(code, msg, func, args)
+            raise MultiFitterException("%s failed (code %d): %s" % (func.__name__, code, msg))
+        return result
+
+    for name in names:
+        if hasattr(lib, name):
+            getattr(lib, name).errcheck = errcheck
     return lib
 
 
@@ -413,6 +456,15 @@ class MultiFitterGaussian(MultiFitter):
             raise MultiFitterException("daoInitialize failed (no usable GPU, or bad sizes)")
         if self.team:
             self.clib.daoSetTeam(self.mfit, self.team)
+
+    def setWaitLimit(self, polls):
+        """
+        Not in the reference: polls after which a scheduler wait of the
+        fitter gives up (0: the default).  A wait that gives up makes
+        iterate() / doFit() raise MultiFitterException and drops the
+        frame's peaks.
+        """
+        self.clib.daoSetWaitLimit(self.mfit, int(polls))
 
     def setTeam(self, peaks_per_workgroup):
         """

@@ -16,7 +16,12 @@ it accepts -- and runs continuous batching over it:
   dao_fit_c.py:332-356) and hands back the results; while one worker fits,
   the others load and fit the next requests, so work from different
   requests overlaps on the device;
-- latency = result ready - arrival, per request.
+- latency = result ready - arrival, per request;
+- failures stay with their request: a frame with non-finite or non-positive
+  pixels (the MLE deviance takes their logarithm), a non-finite background
+  or start peak is refused at submit (that request is done with an error),
+  and a frame the fitter reports as failed (FrameFitError) fails only its
+  own request; the rest of the launch returns its results.
 
 Frames share no state in the fitter, so a frame's result does not depend on
 which other frames share its launch (to the parity bar; bit-identical unless
@@ -70,6 +75,29 @@ def pool_size(workers, slots, max_shape, max_peaks, hbm_budget_gb=None, roi_size
     return max(1, min(workers, fit // slots)), slots
 
 
+def _all_finite(a, positive=False):
+    if hasattr(a, "is_cuda"):
+        import torch
+        ok = torch.isfinite(a).all()
+        if positive:
+            ok = ok & (a > 0).all()
+        return bool(ok)
+    a = numpy.asarray(a)
+    ok = numpy.isfinite(a).all()
+    return bool(ok and (not positive or (a > 0).all()))
+
+
+def _invalid_input(image, background, rows):
+    """Why a request cannot be fitted, or None."""
+    if not _all_finite(image, positive=True):
+        return "the frame holds non-finite or non-positive pixels"
+    if not _all_finite(background):
+        return "the background holds non-finite values"
+    if rows is not None and rows.shape[0] and not _all_finite(rows):
+        return "a start peak holds non-finite values"
+    return None
+
+
 class FitRequest(object):
     """One frame to fit; `result` (peaks dict, as BatchFitter.getPeaks) once `done` is set."""
     def __init__(self, image, background, rows, peaks_type, camera, t_arrival):
@@ -108,7 +136,7 @@ class FitServer(object):
     roi_size, wx_params, ...).
     """
     def __init__(self, workers=4, slots=32, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None,
-                 max_iterations=2000, team=0, margin=-1, **fitter_kw):
+                 max_iterations=2000, team=0, margin=-1, wait_limit=0, **fitter_kw):
         self.max_shape = tuple(max_shape)
         self.max_peaks = max_peaks
         self.max_iterations = max_iterations
@@ -121,6 +149,7 @@ class FitServer(object):
                         for _ in range(workers)]
         for bf in self.fitters:
             bf.setTuning(0, margin, team)
+            bf.setWaitLimit(wait_limit)
         self.queue = queue.Queue()
         self.completed = []
         self._lock = threading.Lock()
@@ -138,8 +167,22 @@ class FitServer(object):
             raise MultiFitterException("%d peaks exceed the slot's %d" % (rows.shape[0], self.max_peaks))
         r = FitRequest(image, background, rows, peaks_type, camera,
                        time.perf_counter() if t_arrival is None else t_arrival)
+        bad = _invalid_input(image, background, rows)
+        if bad is not None:
+            r.error = MultiFitterException("request refused: " + bad)
+            r.t_start = r.t_done = time.perf_counter()
+            r.batch_size = 0
+            with self._lock:
+                self.completed.append(r)
+            r.done.set()
+            return r
         self.queue.put(r)
         return r
+
+    def setWaitLimit(self, polls):
+        """Polls after which a scheduler wait gives up and fails its frame's request (0: default)."""
+        for bf in self.fitters:
+            bf.setWaitLimit(polls)
 
     def _work(self, w):
         bf = self.fitters[w]
@@ -160,6 +203,7 @@ class FitServer(object):
                 batch.append(r)
             t0 = time.perf_counter()
             try:
+                failed = set()
                 for r in batch:
                     r.t_start = t0
                 # one load call per peak format (normally one)
@@ -169,18 +213,28 @@ class FitServer(object):
                                   [batch[k].rows for k in ks], ptype)
                 bf.clearFrames(list(range(len(batch), used)))
                 used = len(batch)
-                bf.doFit(self.max_iterations)
+                try:
+                    bf.doFit(self.max_iterations)
+                except dao_batch.FrameFitError as e:
+                    # only these slots' requests fail; the others were fitted
+                    failed = set(e.frames)
+                    for k in failed:
+                        if k < len(batch):
+                            batch[k].error = e
                 rows, status, counts = bf.getAllPeaks()
                 at = numpy.concatenate([[0], numpy.cumsum(counts)])
                 for k, r in enumerate(batch):
+                    if k in failed:
+                        continue
                     a, b = int(at[k]), int(at[k + 1])
                     res = {c: rows[a:b, i].copy() for i, c in enumerate(dao_batch.PEAK_COLUMNS)}
                     res["iterations"] = res["iterations"].astype(numpy.int32)
                     res["status"] = status[a:b].copy()
                     r.result = res
-            except Exception as e:  # reported to every request of the batch
+            except Exception as e:  # the launch itself failed: every request of it
                 for r in batch:
-                    r.error = e
+                    if r.error is None:
+                        r.error = e
             t1 = time.perf_counter()
             with self._lock:
                 self.launches += 1

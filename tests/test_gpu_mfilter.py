@@ -64,9 +64,9 @@ def test_memo_sequence_matches_oracle():
 
 
 @pytest.mark.gpu
-def test_batched_convolution_matches_oracle():
+@pytest.mark.parametrize("shape", [(128, 96), (33, 47), (512, 512)])
+def test_batched_convolution_matches_oracle(shape):
     import torch
-    shape = (128, 96)
     psf = MO.gaussian_psf(shape, 2.0)
     flt = dao_mfilter.MatchedFilter(psf)
     orc = MO.MatchedFilterOracle(psf)

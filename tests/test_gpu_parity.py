@@ -162,7 +162,9 @@ def test_z_headline_frames_deviations_are_within_the_oracles_own_conditioning(fr
     numpy.testing.assert_array_equal(g["status"], r["status"])
     d = numpy.hypot(g["x"] - r["x"], g["y"] - r["y"])
     off = numpy.nonzero(d > TOL_POS)[0]
-    assert 0 < off.size <= 3, off
+    assert off.size <= 3, off
+    if off.size == 0:
+        return
     spread = numpy.zeros(off.size)
     rng = numpy.random.default_rng(1)
     for t in range(4):
