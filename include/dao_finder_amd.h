@@ -63,6 +63,23 @@ int daoFindLocalMaxima(const double *images, int32_t *taken, int zsize, int ysiz
                        int z_range, double radius, double threshold, const double *z_values, int max_peaks,
                        double *z, double *y, double *x, double *h, int *n_found);
 
+/* daoFindLocalMaxima that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same search and results as daoFindLocalMaxima, with:
+ * z_values (zsize doubles) in device memory; counts (2 ints, device memory)
+ * receiving {n_found, maxima before the max_peaks cut}; the work queued on
+ * `stream` (NULL: the null stream) and not waited for.  The scratch is a
+ * fixed device array of 17 MiB that each GPU holds until the process ends:
+ * calls with more than 2^24 pixels (zsize * ysize * xsize), 65536 rows
+ * (zsize * ysize) or 4096 planes are refused with -1 and a message.  Every
+ * call on one GPU uses that GPU's scratch: calls that overlap on one GPU
+ * (other streams, other threads) race.  0 on success, -1 bad arguments, -2
+ * device error. */
+int daoFindLocalMaximaNoAlloc(const double *images, int32_t *taken, int zsize, int ysize, int xsize, int margin,
+                              int z_range, double radius, double threshold, const double *z_values, int max_peaks,
+                              double *z, double *y, double *x, double *h, int *counts, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

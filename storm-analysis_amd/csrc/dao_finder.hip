@@ -22,6 +22,8 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <mutex>
+
 #include "../../include/dao_finder_amd.h"
 
 namespace {
@@ -197,6 +199,46 @@ __global__ __launch_bounds__(kThreads) void k_emit(FlmArgs a) {
   }
 }
 
+// n_found = min(maxima, max_peaks) and the maxima count, on the device.
+__global__ void k_counts(const int* tot, int zs, int max_peaks, int* out) {
+  const int total = tot[zs];
+  out[0] = total < max_peaks ? total : (max_peaks > 0 ? max_peaks : 0);
+  out[1] = total;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point: a file-scope device array of
+// fixed size (every GPU the library runs on holds one until the process
+// ends).  Host code never uses the array's name as an address (on the host
+// it is a stand-in symbol); each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+// k_mark writes every mask byte and row count of the call's frames before
+// k_scan_rows / k_scan / k_emit read them, so nothing needs clearing.
+constexpr long long kNoAllocMaxPixels = 1ll << 24;  // mask bytes: zsize * ysize * xsize
+constexpr long long kNoAllocMaxRows = 1ll << 16;    // row counts: zsize * ysize
+constexpr int kNoAllocMaxPlanes = 4096;             // plane offsets: zsize + 1
+constexpr long long kNoAllocCountsOff = kNoAllocMaxPixels;
+constexpr long long kNoAllocBytes = kNoAllocMaxPixels + 4 * (kNoAllocMaxRows + kNoAllocMaxPlanes + 64);
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) uint8_t scratch_noalloc[kNoAllocBytes];
+
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+// This device's address of scratch_noalloc (nullptr on failure).
+void* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (g_noalloc_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch_noalloc)) != hipSuccess) return nullptr;
+    g_noalloc_ptr[dev] = p;
+  }
+  return g_noalloc_ptr[dev];
+}
+
 hipStream_t stream() {
   static hipStream_t st = nullptr;
   if (st == nullptr && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = nullptr;
@@ -284,6 +326,54 @@ int daoFindLocalMaxima(const double* images, int32_t* taken, int zsize, int ysiz
   const int rc = find_device(images, taken, zsize, ysize, xsize, margin, z_range, radius, threshold, z_values,
                              max_peaks, z, y, x, h, n_found, st);
   return rc < 0 ? rc : 0;
+}
+
+// daoFindLocalMaxima with no allocation: the scratch is this device's
+// scratch_noalloc, z values and counts live in device memory, and the work
+// is queued on the caller's stream without a host synchronisation.
+int daoFindLocalMaximaNoAlloc(const double* images, int32_t* taken, int zsize, int ysize, int xsize, int margin,
+                              int z_range, double radius, double threshold, const double* z_values, int max_peaks,
+                              double* z, double* y, double* x, double* h, int* counts, void* stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!args_ok(zsize, ysize, xsize, z_range, radius) || images == nullptr || taken == nullptr ||
+      z_values == nullptr || counts == nullptr ||
+      (max_peaks > 0 && (z == nullptr || y == nullptr || x == nullptr || h == nullptr)))
+    return -1;
+  const long long npx = (long long)zsize * ysize * xsize, nrows = (long long)zsize * ysize;
+  if (npx > kNoAllocMaxPixels || nrows > kNoAllocMaxRows || zsize > kNoAllocMaxPlanes) {
+    fprintf(stderr,
+            "daoFindLocalMaximaNoAlloc: %d x %d x %d pixels exceed the scratch (at most %lld pixels, %lld rows, "
+            "%d planes)\n",
+            zsize, ysize, xsize, kNoAllocMaxPixels, kNoAllocMaxRows, kNoAllocMaxPlanes);
+    return -1;
+  }
+  uint8_t* s = static_cast<uint8_t*>(noalloc_scratch());
+  if (s == nullptr) return -2;
+  FlmArgs a;
+  a.img = images;
+  a.taken = taken;
+  a.zs = zsize;
+  a.ys = ysize;
+  a.xs = xsize;
+  a.margin = margin;
+  a.z_range = z_range;
+  a.radius = radius;
+  a.threshold = threshold;
+  a.flag = s;
+  a.rowcnt = reinterpret_cast<int*>(s + kNoAllocCountsOff);
+  a.zv = z_values;
+  a.oz = z;
+  a.oy = y;
+  a.ox = x;
+  a.oh = h;
+  a.max_peaks = max_peaks;
+  const dim3 grid((unsigned)ysize, (unsigned)zsize);
+  k_mark<false><<<grid, kThreads, 0, st>>>(a);
+  k_scan_rows<<<zsize, kThreads, 0, st>>>(a.rowcnt, ysize, a.rowcnt + nrows);
+  k_scan<<<1, kThreads, 0, st>>>(a.rowcnt + nrows, zsize);
+  if (max_peaks > 0) k_emit<<<grid, kThreads, 0, st>>>(a);
+  k_counts<<<1, 1, 0, st>>>(a.rowcnt + nrows, zsize, max_peaks, counts);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int calcMaxPeaks(flmData* f) {
