@@ -56,11 +56,14 @@ struct FilterArgs {
   double rb;          // r_neighbors (dimmer, low significance)
 };
 
-// The kd-tree's squared distance of point k from query i, unfused.
+// The kd-tree's squared distance of point k from query i, unfused
+// (contraction off: the __d*_rn intrinsics are plain operators without
+// OCML's rounded operations, so they would not stop an FMA).
 __device__ inline double dist_sq(double xk, double yk, double xi, double yi) {
+#pragma clang fp contract(off)
   const double dx = xk - xi;
   const double dy = yk - yi;
-  return __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+  return dx * dx + dy * dy;
 }
 
 __device__ inline double wave_min(double v) {

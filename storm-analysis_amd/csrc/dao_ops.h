@@ -380,6 +380,28 @@ __device__ void journal_unwind(const Pix& px, const Frame& fr, const R& c, const
   }
 }
 
+// One pixel of mFitAddPeak / mFitSubtractPeak (multi_fit.c:50-55,
+// 1916-1920): f +/-= (h psf) rqe and bg +/-= (b rqe) + scmos, each operation
+// rounded on its own in the reference's order.  Contraction is off here:
+// left to the compiler, the product and the sum fuse into an FMA, the
+// subtraction then yields the rounding error of the product instead of
+// cancelling the addition, and a peak taken out of the fit leaves a residue
+// where the reference leaves exact zeros.  (The __d*_rn intrinsics do not
+// prevent that: without OCML's rounded operations they are plain operators.)
+__device__ inline void stamp_pixel(double& f, double& bg, double h, double psf, double b, double r, double sc,
+                                   int sign) {
+#pragma clang fp contract(off)
+  const double a = (h * psf) * r;
+  const double t = b * r + sc;
+  if (sign > 0) {
+    f = f + a;
+    bg = bg + t;
+  } else {
+    f = f - a;
+    bg = bg - t;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // ROI cache of one wave
 // ---------------------------------------------------------------------------
@@ -523,15 +545,8 @@ struct Roi {
       if (ok(s)) {
         double r = rqe(s);
         double sc = scm(s);
-        if (sign > 0) {
-          f[s] += h * psf[s] * r;
-          cnt[s] += 1;
-          bg[s] += b * r + sc;
-        } else {
-          f[s] -= h * psf[s] * r;
-          cnt[s] -= 1;
-          bg[s] -= (b * r + sc);
-        }
+        stamp_pixel(f[s], bg[s], h, psf[s], b, r, sc, sign);
+        cnt[s] += sign > 0 ? 1 : -1;
       }
     }
     stale = okm;
@@ -891,16 +906,8 @@ struct RoiM {
       if (ok(s)) {
         const double r = (double)rqe(s);
         const double sc = (double)scm(s);
-        const double a = h * psf[s] * r;
-        if (sign > 0) {
-          f[s] += a;
-          cnt[s] += 1;
-          bg[s] += b * r + sc;
-        } else {
-          f[s] -= a;
-          cnt[s] -= 1;
-          bg[s] -= (b * r + sc);
-        }
+        stamp_pixel(f[s], bg[s], h, psf[s], b, r, sc, sign);
+        cnt[s] += sign > 0 ? 1 : -1;
       }
     }
     stale = okm;

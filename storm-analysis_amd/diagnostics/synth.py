@@ -118,6 +118,13 @@ def frame_2dfixed(rng, size=256, n=200, sigma=1.5, bg=100.0, hmin=500.0, hmax=30
     return Frame(img, peaks, {"x": x, "y": y, "height": h, "sigma": s})
 
 
+def z_background(shape):
+    """The Z workload's background, 125 + 75 sin(2 pi x / W) cos(2 pi y / H), in [50, 200]."""
+    h, w = shape
+    jj, ii = numpy.meshgrid(numpy.arange(w), numpy.arange(h))
+    return 125.0 + 75.0 * numpy.sin(2 * numpy.pi * jj / w) * numpy.cos(2 * numpy.pi * ii / h)
+
+
 def frame_z(rng, size=512, n=1000):
     """
     3D-astigmatism workload: z ~ U(-0.5,0.5) um, widths from WX_Z/WY_Z,

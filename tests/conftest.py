@@ -18,7 +18,9 @@ import __graft_entry__  # noqa: E402
 __graft_entry__.load_package()
 from storm_analysis_amd.sa_library import dao_fit_c  # noqa: E402
 
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_dao.so")
+# DAO_ORACLE_SO points the oracle back end at another build of the oracle
+# (test_oracle_mutations.py runs the pins against deliberately broken copies)
+ORACLE_SO = os.environ.get("DAO_ORACLE_SO", os.path.join(ROOT, "oracle", "liboracle_dao.so"))
 HIP_SO = os.path.join(ROOT, "storm-analysis_amd", "c_libraries", "libdao_fit.so")
 
 

@@ -53,6 +53,20 @@ def test_prepare_and_snr_bit_identical(with_maps):
 
 
 @pytest.mark.gpu
+def test_prepare_and_snr_hand_values():
+    """The hand-worked values of test_mfilter_oracle.FINDER_HAND through the HIP kernels."""
+    from test_mfilter_oracle import FINDER_HAND, finder_hand
+    bg_var, fg = PF.finderPrepare(cuda(finder_hand("image")[None]), cuda(finder_hand("background")[None]),
+                                  cuda(finder_hand("fit")[None]), finder_hand("camera"))
+    numpy.testing.assert_allclose(bg_var[0].cpu().numpy(), finder_hand("bg_var"), rtol=1e-15, atol=1e-15)
+    numpy.testing.assert_allclose(fg[0].cpu().numpy(), finder_hand("foreground"), rtol=1e-15, atol=1e-15)
+    snr = PF.finderSNR(fg, bg_var, finder_hand("mask")).cpu().numpy()[0]
+    numpy.testing.assert_allclose(snr, finder_hand("snr"), rtol=1e-14)
+    plain = PF.finderSNR(fg, bg_var).cpu().numpy()[0]
+    assert abs(plain[2, 1] - FINDER_HAND["snr_unmasked_last"]) < 1e-13
+
+
+@pytest.mark.gpu
 def test_search_image_through_the_matched_filters():
     shape = (96, 80)
     image, background, fit = inputs(5, 3, shape)

@@ -103,3 +103,52 @@ def test_memo_returns_the_psf_before_any_convolution():
     psf = MO.gaussian_psf((16, 20), 2.0)
     flt = MO.MatchedFilterOracle(psf, memoize=True, max_diff=0.1)
     numpy.testing.assert_array_equal(flt.convolve(numpy.full((16, 20), 0.05)), MO.recenter_psf(psf))
+
+
+# Hand values for the finder's per-pixel arithmetic (fitting.py:452-494),
+# worked out by hand from
+#   image      [[10, 5], [3, 1], [4, 6]]
+#   background [[2, 2], [1, 0.05], [-1, 0.5]]
+#   fit        [[1, 0], [0, 0], [0.5, -0.4]]
+#   camera var [[0.5, 0], [0, 0.02], [0.1, 0.05]]
+# bg_var = background + fit + camera var:
+#   [[3.5, 2], [1, 0.07], [-0.4, 0.15]]
+# foreground = image - background - fit:
+#   [[7, 3], [2, 0.95], [4.5, 5.9]]
+# bg_var clamped where <= 0.1 (0.07 and -0.4 -> 0.1; 0.15 stays), then
+# snr = foreground / sqrt(bg_var) * mask with mask [[1, 1], [1, 1], [1, 0]]:
+#   7/sqrt(3.5) = 3.7416573867739413   3/sqrt(2) = 2.1213203435596424
+#   2/sqrt(1) = 2                      0.95/sqrt(0.1) = 3.0041637771599603
+#   4.5/sqrt(0.1) = 14.230249470757707 5.9/sqrt(0.15) * 0 = 0
+FINDER_HAND = {
+    "image": [[10.0, 5.0], [3.0, 1.0], [4.0, 6.0]],
+    "background": [[2.0, 2.0], [1.0, 0.05], [-1.0, 0.5]],
+    "fit": [[1.0, 0.0], [0.0, 0.0], [0.5, -0.4]],
+    "camera": [[0.5, 0.0], [0.0, 0.02], [0.1, 0.05]],
+    "mask": [[1.0, 1.0], [1.0, 1.0], [1.0, 0.0]],
+    "bg_var": [[3.5, 2.0], [1.0, 0.07], [-0.4, 0.15]],
+    "foreground": [[7.0, 3.0], [2.0, 0.95], [4.5, 5.9]],
+    "snr": [[3.7416573867739413, 2.1213203435596424], [2.0, 3.0041637771599603], [14.230249470757707, 0.0]],
+    # without the mask the last pixel is 5.9 / sqrt(0.15)
+    "snr_unmasked_last": 15.233734495082507,
+}
+
+
+def finder_hand(name):
+    return numpy.array(FINDER_HAND[name], dtype=numpy.float64)
+
+
+def test_finder_arithmetic_hand_values():
+    """
+    The clamp replaces every variance <= 0.1 (including negative ones) by
+    0.1 and leaves 0.15 alone.  Plausible slips that fail: no clamp (NaN at
+    -0.4), clamping the standard deviation at 0.1 instead of the variance,
+    the camera variance left out, the fit image added to the foreground.
+    """
+    bg_var, fg = MO.finder_prepare(finder_hand("image"), finder_hand("background"), finder_hand("fit"),
+                                   finder_hand("camera"))
+    numpy.testing.assert_allclose(bg_var, finder_hand("bg_var"), rtol=1e-15, atol=1e-15)
+    numpy.testing.assert_allclose(fg, finder_hand("foreground"), rtol=1e-15, atol=1e-15)
+    snr = MO.finder_snr(fg, bg_var, finder_hand("mask"))
+    numpy.testing.assert_allclose(snr, finder_hand("snr"), rtol=1e-14)
+    assert abs(MO.finder_snr(fg, bg_var)[2, 1] - FINDER_HAND["snr_unmasked_last"]) < 1e-13
