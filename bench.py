@@ -11,15 +11,18 @@ One step = newImage + newBackground + newPeaks ("hdf5" start peaks) + doFit
 the host, for every frame of the batch; images and start peaks are resident
 in HBM when the timed region starts.
 
-`value` is the batched converged fits/s in mixed precision (FP32 deviance
-terms, Jacobian and Hessian; FP64 peak shape, fit-image accumulators,
-deviance sums, Cholesky solve and accept/reject), whose accuracy against FP64 is
-reported beside it (all frames against the FP64 GPU path, the CPU sample
-against the oracle).  Also reported: the batched FP64 path, the plain path
-(MultiFitterZ, one frame at a time through the single-frame C-ABI, FP64, host
-arrays in and out) with its p50/p99 frame latency, the fit kernel's HBM
-roofline (algorithmic bytes = LM cycles x N_roi x 36 B over the kernel's
-event-timed duration), and the oracle timed on the host's cores.
+`value` is the batched converged fits/s in FP64, the reference's arithmetic
+(its results match the oracle to the parity bar).  Beside it: the same in
+mixed precision (FP32 deviance terms, Jacobian and Hessian; FP64 peak shape,
+fit-image accumulators, deviance sums, Cholesky solve and accept/reject) with
+its accuracy against FP64 and the oracle; for each precision two
+host-to-host rates (timed from the upload of host frames to the results on
+the host: float64 frames, and raw uint16 camera frames converted on the GPU
+by daoIngestFrames); the plain path (MultiFitterZ, one frame at a time
+through the single-frame C-ABI, FP64, host arrays in and out) with its
+p50/p99 frame latency; the fit kernel's HBM roofline (algorithmic bytes = LM
+cycles x N_roi x 36 B over the kernel's event-timed duration); and the
+oracle timed on the host's cores.
 """
 import argparse
 import concurrent.futures as cf
@@ -72,6 +75,7 @@ def parse():
     p.add_argument("--plain-frames", type=int, default=20)
     p.add_argument("--cpu-frames", type=int, default=500)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-host", action="store_true", help="skip the host-to-host rates")
     p.add_argument("--margin", type=int, default=2)
     p.add_argument("--no-ingest", action="store_true", help="skip the frame-ingest leg")
     p.add_argument("--no-filter", action="store_true", help="skip the post-fit filter leg")
@@ -155,10 +159,11 @@ def cpu_baseline(images, backgrounds, rows, counts, nframes, passes=2, single_fr
     from storm_analysis_amd.sa_library import dao_fit_c as D
     lib = D.setDaoFitArgtypes(ctypes.cdll.LoadLibrary(os.path.join(ROOT, "oracle", "liboracle_dao.so")))
     try:
-        cores = len(os.sched_getaffinity(0))
+        visible = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        visible = os.cpu_count() or 1
+    # a GPU host allots 16 CPUs to each GPU; use them all, but no more
+    cores = max(1, min(visible, 16))
 
     fac = lambda: D.MultiFitterZ(roi_size=ROI, **zkw())  # noqa: E731
 
@@ -179,13 +184,52 @@ def cpu_baseline(images, backgrounds, rows, counts, nframes, passes=2, single_fr
         dt1 = time.perf_counter() - t1
     conv = passes * sum(r[0] for r in res)
     return {"value": conv / dt, "unit": "converged fits/s", "cores": cores, "kind": "port",
+            "host_cpus_visible": visible, "os_cpu_count": os.cpu_count(),
             "sample": "%d passes over %d of the workload's frames (%d converged fits) in %.1f s on %d threads" %
                       (passes, nframes, conv, dt, cores),
             "one_thread": {"value": conv1 / dt1, "frames": min(single_frames, nframes), "seconds": dt1}}, \
         [r[1] for r in res]
 
 
-def run_batched(precision, args, images, backgrounds, rows, counts_np):
+def host_steps(bf, precision, args, host, counts_np):
+    """
+    Host-to-host rates of the batch: per step the frames, backgrounds and
+    start peaks are uploaded from (pinned) host memory, fitted, and the peak
+    properties copied back to the host.  Two forms: float64 frames; raw uint16
+    camera frames (ADU = photo-electrons + 100 offset, gain 1, as generated)
+    converted to photo-electrons on the GPU by daoIngestFrames before the fit.
+    """
+    import torch
+    from storm_analysis_amd.sa_library import dao_fit_c as D
+    from storm_analysis_amd.sa_library.dao_ingest import FrameReaderStd
+    reader = FrameReaderStd(camera_gain=1.0, camera_offset=100.0, device="cuda:0")
+    out = {}
+    for form in ("float64", "uint16"):
+        def step():
+            if form == "float64":
+                bf.newImages(host["images"])
+            else:
+                bf.newImages(reader.loadFrames(host["raw"]))
+            bf.newBackgrounds(host["backgrounds"])
+            bf.newPeaks(host["rows"], "hdf5", counts=counts_np)
+            bf.doFit()
+            return bf.getAllPeaks()
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        conv = 0
+        for _ in range(args.steps):
+            _, status, _ = step()
+            conv += int((status == D.CONVERGED).sum())
+        dt = time.perf_counter() - t0
+        out[form] = {"fits_per_s": conv / dt, "ms_per_step": dt * 1e3 / args.steps,
+                     "upload_bytes_per_step": int(host["bytes"][form])}
+        log("%s host-to-host (%s frames): %.1f ms/step, %.0f converged fits/s" %
+            (precision, form, dt * 1e3 / args.steps, conv / dt))
+    return out
+
+
+def run_batched(precision, args, images, backgrounds, rows, counts_np, host=None):
     """Time args.steps batched steps in one precision; returns the measurements and results."""
     import torch
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D
@@ -229,9 +273,11 @@ def run_batched(precision, args, images, backgrounds, rows, counts_np):
                                                       mt["waves"]):
         traffic = 1024.0 * (mt["fetch_kib"] * mt["fetch_scale"] + mt["write_kib"] * mt["write_scale"])
     st = bf.stats()
-    bf.cleanup()
+    bf.setTiming(False)
     log("%s: %.1f ms/step, %d converged per step, fit kernel %.1f ms per launch" %
         (precision, dt * 1e3 / args.steps, conv // max(1, args.steps), lm_ms / max(1, launches)))
+    host_to_host = host_steps(bf, precision, args, host, counts_np) if host is not None else None
+    bf.cleanup()
     return {
         "fits_per_s": conv / dt,
         "ms_per_step": dt * 1e3 / args.steps,
@@ -245,6 +291,7 @@ def run_batched(precision, args, images, backgrounds, rows, counts_np):
                      "avg_launch_us": (lm_ms * 1e3 / launches) if launches else None,
                      "lm_cycles": cycles, "bytes_per_cycle": N_ROI_18 * ALG_BYTES_PER_PIXEL},
         "scheduler": {"sweeps": st["sweeps"], "escape_reruns": st["escape_reruns"]},
+        "host_to_host": host_to_host,
     }, (out, status, per_frame)
 
 
@@ -315,6 +362,10 @@ def uneven_leg(log, seed=20261209):
             b = out["autotune"]["best"]
             bf.setTuning(b["waves"], b["margin"], b["team"])
         prepare()
+        # counters since the last read: drop what the autotuner's trial fits
+        # left, so lm_cycles counts the timed fit only
+        for k in range(len(frames)):
+            bf.getCounters(k)
         s0 = bf.stats()
         t0 = time.perf_counter()
         bf.doFit()
@@ -757,10 +808,20 @@ def main():
     t_gen = time.perf_counter() - t_gen
     counts_np = counts.numpy()
 
-    precisions = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    # host copies of the inputs for the host-to-host rates (pinned, as a
+    # camera pipeline's staging buffers would be)
+    host = None
+    if not args.no_host:
+        raw = (images + 100.0).to(torch.uint16)
+        host = {"images": images.cpu().pin_memory(), "backgrounds": backgrounds.cpu().pin_memory(),
+                "rows": rows.cpu().pin_memory(), "raw": raw.cpu().pin_memory()}
+        extra = host["backgrounds"].numel() * 8 + host["rows"].numel() * 8
+        host["bytes"] = {"float64": host["images"].numel() * 8 + extra, "uint16": host["raw"].numel() * 2 + extra}
+        del raw
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     meas, res = {}, {}
     for p in precisions:
-        meas[p], res[p] = run_batched(p, args, images, backgrounds, rows, counts_np)
+        meas[p], res[p] = run_batched(p, args, images, backgrounds, rows, counts_np, host)
     head = precisions[0]
 
     # plain path: one frame at a time through the single-frame C-ABI (FP64)
@@ -821,7 +882,7 @@ def main():
 
     serving = None
     if args.serve_seconds > 0:
-        serving = serve_leg(args, head, log)
+        serving = serve_leg(args, "mixed" if args.precision == "both" else head, log)
 
     ingest = None if args.no_ingest else ingest_leg(log)
     filters = None if args.no_filter else filter_leg(log)
@@ -840,7 +901,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "mixed: fp32 deviance/J/H terms, fp64 shape/accumulators/solve/chi2" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "fp64" else "mixed: fp32 deviance/J/H terms, fp64 shape/accumulators/solve/chi2",
         "data": "synthetic (seeded GPU generator: Poisson frames, offset 100 ADU, gain 1)",
         "config": {"workload": "3D astigmatic Z model, %d frames of %dx%d, %d emitters/frame, ROI %d, MLE" %
                                (F, S, S, N, ROI),
@@ -849,10 +910,13 @@ def main():
         "roofline": hm["roofline"],
         "cpu_baseline": cpu,
         "batched": meas,
+        "host_to_host": {p: meas[p].get("host_to_host") for p in meas},
         "plain": {"fits_per_s": plain_fps, "frames": args.plain_frames,
                   "p50_ms": float(numpy.percentile(lat_ms, 50)), "p99_ms": float(numpy.percentile(lat_ms, 99)),
                   "path": "MultiFitterZ, one frame per call, FP64"},
         "batched_vs_plain": hm["fits_per_s"] / plain_fps if plain_fps > 0 else None,
+        "batched_vs_plain_by_precision": {p: meas[p]["fits_per_s"] / plain_fps if plain_fps > 0 else None
+                                          for p in meas},
         "frame_latency_p99_ms": float(numpy.percentile(lat_ms, 99)),
         "single_frame_latency": latency,
         "accuracy": accuracy,
