@@ -208,4 +208,28 @@ struct FitArgs {
   const int* team_seat0;        // [F + 1]: frame f's seats are [team_seat0[f], team_seat0[f + 1])
 };
 
+// The multiplane fitter (k_fit_mp): frame-sets of nch channels, channel c of
+// set k in frame set_frame0[k] + c; peak i of a set is slot peak0 + i of each
+// channel's frame.  cpl: the coupling table in device memory, in this order:
+// xt_0toN[3 nch], yt_0toN[3 nch], xt_Nto0[3 nch], yt_Nto0[3 nch],
+// width_min[nch], width_max[nch], w_x[n_weights nch], w_y[n_weights nch].
+constexpr int kMpMaxChannels = 8;
+struct MpArgs {
+  Model m;  // 2D Gaussian, MLE; widths per channel from cpl
+  Pix px;
+  Peaks pk;
+  const Frame* frames;
+  const int* roi_tab;
+  int nch;
+  int nsets;
+  const int* set_frame0;
+  const double* cpl;
+  int n_weights;
+  double w_z_offset, w_z_scale;
+  double tol;
+  int max_more;         // doFit's max_iterations; 0 = exactly one sweep
+  int* frame_counters;  // [F][C_COUNT]
+  int* frame_sweeps;    // [F]: sweeps run, at each set's channel-0 frame
+};
+
 }  // namespace dao

@@ -274,13 +274,43 @@ void Engine::grow_total_cap(long long c) {
 
 void Engine::ensure_frame_cap(int f, int need) {
   if (F != 1) {
-    if (need > fr[f].cap) fail(kErrCapacity, "more peaks than the frame's capacity");
+    if (need <= fr[f].cap) return;
+    if (!growable) fail(kErrCapacity, "more peaks than the frame's capacity");
+    grow_frames(std::max({need, 2 * fr[f].cap, 512}));
     return;
   }
   if (need <= fr[0].cap && need <= cap) return;
   long long c = std::max<long long>({(long long)need, 2 * cap, 512});
   fr[0].cap = (int)c;
   grow_total_cap(c);
+  upload_frames();
+}
+
+// Every frame gets `newcap` peak slots (frame f at [f newcap, (f + 1)
+// newcap)); each frame's live peaks keep their order.
+void Engine::grow_frames(int newcap) {
+  const long long total = (long long)F * newcap;
+  Peaks np{};
+  np.cap = total;
+  np.rec = dalloc<PeakRec>(total);
+  np.bk = dalloc<PeakRec>(total);
+  for (int f = 0; f < F; f++)
+    if (fr[f].nfit > 0)
+      DAO_CHECK(hipMemcpyAsync(np.rec + (long long)f * newcap, pk.rec + fr[f].peak0, sizeof(PeakRec) * fr[f].nfit,
+                               hipMemcpyDeviceToDevice, st));
+  DAO_CHECK(hipStreamSynchronize(st));
+  dfree(pk.rec);
+  dfree(pk.bk);
+  free_sched();
+  pk = np;
+  alloc_sched(total);
+  cap = total;
+  for (int f = 0; f < F; f++) {
+    fr[f].peak0 = (int)((long long)f * newcap);
+    fr[f].cap = newcap;
+    // empty only the slots past the live records copied above
+    launch_set_owner(pk, (long long)f * newcap + fr[f].nfit, (long long)(f + 1) * newcap, f, st);
+  }
   upload_frames();
 }
 
@@ -727,6 +757,14 @@ void Engine::set_status(int f, const int32_t* status) {
 // ---------------------------------------------------------------------------
 // read-back
 // ---------------------------------------------------------------------------
+
+void Engine::read_counters(std::vector<int>& out, int f) {
+  int* h = pin(sizeof(int) * C_COUNT);
+  DAO_CHECK(hipMemcpyAsync(h, d_counters + (size_t)f * C_COUNT, sizeof(int) * C_COUNT, hipMemcpyDeviceToHost, st));
+  launch_fill_i32(d_counters + (size_t)f * C_COUNT, C_COUNT, 0, st);
+  sync();
+  out.assign(h, h + C_COUNT);
+}
 
 void Engine::read_counters(std::vector<int>& out) {
   size_t n = (size_t)F * C_COUNT;

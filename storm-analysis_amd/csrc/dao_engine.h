@@ -98,6 +98,7 @@ void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
 void launch_iota(int* p, int n, int base, hipStream_t st);
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st);
 void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st);
+void launch_fit_mp(const MpArgs& a, hipStream_t st);  // dao_mp.hip
 void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st);
 void launch_restore_bad_frames(const Frame* frames, int nframes, const int* bad, PixRW* rw, const PixRW* rw_copy,
                                PeakRec* rec, const PeakRec* rec_copy, hipStream_t st);
@@ -159,7 +160,11 @@ class Engine {
   void anscombe(int f);
   void set_calibration(int f, const double* rqe_host, const double* scmos_host);
   // peaks
-  void ensure_frame_cap(int f, int need);  // single-frame growth (F == 1 only)
+  // room for `need` peaks in frame f: a single-frame engine (and one marked
+  // growable) grows its peak storage, any other refuses
+  void ensure_frame_cap(int f, int need);
+  void grow_frames(int newcap);  // every frame to newcap slots
+  bool growable = false;
   void new_peaks(int f, const double* rows_host, int fmt, int n);
   void new_peaks_multi(const std::vector<int>& frames, const std::vector<const double*>& rows_dev,
                        const std::vector<int>& counts, int fmt);
@@ -190,6 +195,7 @@ class Engine {
   long long sweeps_total = 0;
   // read-back
   void read_counters(std::vector<int>& out);  // F * C_COUNT, zeroes the device copy
+  void read_counters(std::vector<int>& out, int f);  // frame f's C_COUNT, zeroes them
   void get_records(int f, std::vector<PeakRec>& out);  // frame f's live peak records
   void get_status(int f, std::vector<int32_t>& out);
   void get_iterations(int f, std::vector<int32_t>& out);

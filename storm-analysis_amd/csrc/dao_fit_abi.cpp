@@ -12,8 +12,7 @@
 #include <memory>
 #include <vector>
 
-#include "dao_engine.h"
-#include "../../include/dao_fit_amd.h"
+#include "dao_handle.h"
 
 using dao::Engine;
 
@@ -21,27 +20,17 @@ namespace {
 
 constexpr int kPeakBlock = 500;  // INCNPEAKS (multi_fit.h:49)
 
-struct Handle {
-  fitData pub;  // must stay first
-  std::unique_ptr<Engine> eng;
-  dao::Model model;
-  std::vector<int> roi_x, roi_y;
-};
+using Handle = dao::FitHandle;
 
-Handle* H(fitData* fd) { return reinterpret_cast<Handle*>(fd); }
+Handle* H(fitData* fd) { return dao::handle_of(fd); }
 
+// Counter increments of the handle's frame; the other frames' increments (a
+// multiplane fitter's other channels) are not read here, so they stay on
+// the device for their own handles.
 void add_counters(Handle* h) {
   std::vector<int> c;
-  h->eng->read_counters(c);
-  fitData& p = h->pub;
-  p.n_dposv += c[dao::C_DPOSV];
-  p.n_iterations += c[dao::C_ITER];
-  p.n_lost += c[dao::C_LOST];
-  p.n_margin += c[dao::C_MARGIN];
-  p.n_neg_fi += c[dao::C_NEG_FI];
-  p.n_neg_height += c[dao::C_NEG_HEIGHT];
-  p.n_non_converged += c[dao::C_NON_CONV];
-  p.n_non_decr += c[dao::C_NON_DECR];
+  h->eng->read_counters(c, h->frame);
+  dao::add_frame_counters(h->pub, c, 0);
 }
 
 void set_model(Handle* h, int shape, int errm, int jac) {
@@ -75,6 +64,12 @@ T run_value(T on_error, Fn&& fn) {
   return v;
 }
 
+// A multiplane channel (a handle that does not own its engine) is fitted
+// through mpIterateLM with its group; a single-channel sweep is refused.
+void refuse_channel_view(Handle* h) {
+  if (!h->own) dao::fail(dao::kErrArg, "a multiplane channel is fitted by mpIterateLM / mpDoFit");
+}
+
 int peak_format(const char* p_type) {
   if (!std::strcmp(p_type, "finder")) return dao::PF_FINDER;
   if (!std::strcmp(p_type, "testing")) return dao::PF_TESTING;
@@ -83,6 +78,41 @@ int peak_format(const char* p_type) {
 }
 
 }  // namespace
+
+namespace dao {
+
+// daoNewPeaks (dao_fit.c:1582-1766) on a handle's frame, with the storage
+// policy of mFitNewPeaks (multi_fit.c:1428-1477): growing drops ERROR peaks
+// and rounds the capacity to blocks of 500.  Throws on failure.
+void handle_new_peaks(FitHandle* h, const double* peak_params, const char* p_type, int n_peaks) {
+  fitData& p = h->pub;
+  const int fmt = peak_format(p_type);
+  if (p.nfit + n_peaks > p.max_nfit) {
+    int n_alloc = kPeakBlock * ((p.nfit + n_peaks) / kPeakBlock + 1);
+    std::vector<int32_t> st, keep;
+    h->eng->get_status(h->frame, st);
+    for (int i = 0; i < (int)st.size(); i++)
+      if (st[i] != ST_ERROR) keep.push_back(i);
+    h->eng->compact(h->frame, keep);
+    p.max_nfit = n_alloc;
+    p.nfit = (int)keep.size();
+  }
+  if (fmt < 0) {
+    // the reference allocates and initialises the slots, then leaves them
+    // untouched (dao_fit.c:1753-1755)
+    std::printf("Unrecognized peak type '%s'!\n", p_type);
+    return;
+  }
+  if (n_peaks > 0) {
+    h->eng->new_peaks(h->frame, peak_params, fmt, n_peaks);
+    p.nfit += n_peaks;
+  }
+  std::vector<int> c;
+  h->eng->read_counters(c, h->frame);
+  add_frame_counters(p, c, 0);
+}
+
+}  // namespace dao
 
 extern "C" {
 
@@ -134,7 +164,9 @@ fitData* daoInitialize(double* rqe, double* scmos_cal, double tol, int im_size_x
   spec[0].H = im_size_y;
   spec[0].cap = 0;
   const int rc = dao::guarded([&] {
-    h->eng.reset(new Engine(m, spec));
+    h->own.reset(new Engine(m, spec));
+    h->eng = h->own.get();
+    h->frame = 0;
     h->eng->set_calibration(0, rqe, scmos_cal);
   });
   if (rc != dao::kOk) {
@@ -187,40 +219,14 @@ void daoInitializeZ(fitData* fd, double* wx, double* wy, double z_min, double z_
 }
 
 void daoNewPeaks(fitData* fd, double* peak_params, char* p_type, int n_peaks) {
-  run([&] {
-    Handle* h = H(fd);
-    fitData& p = h->pub;
-    const int fmt = peak_format(p_type);
-    // storage policy of mFitNewPeaks (multi_fit.c:1428-1477): growing drops
-    // ERROR peaks and rounds the capacity to blocks of 500
-    if (p.nfit + n_peaks > p.max_nfit) {
-      int n_alloc = kPeakBlock * ((p.nfit + n_peaks) / kPeakBlock + 1);
-      std::vector<int32_t> st, keep;
-      h->eng->get_status(0, st);
-      for (int i = 0; i < (int)st.size(); i++)
-        if (st[i] != dao::ST_ERROR) keep.push_back(i);
-      h->eng->compact(0, keep);
-      p.max_nfit = n_alloc;
-      p.nfit = (int)keep.size();
-    }
-    if (fmt < 0) {
-      // the reference allocates and initialises the slots, then leaves them
-      // untouched (dao_fit.c:1753-1755)
-      std::printf("Unrecognized peak type '%s'!\n", p_type);
-      return;
-    }
-    if (n_peaks > 0) {
-      h->eng->new_peaks(0, peak_params, fmt, n_peaks);
-      p.nfit += n_peaks;
-    }
-    add_counters(h);
-  });
+  run([&] { dao::handle_new_peaks(H(fd), peak_params, p_type, n_peaks); });
 }
 
 void daoCleanup(fitData* fd) {
   if (fd == nullptr) return;
   Handle* h = H(fd);
-  h->eng.reset();
+  if (!h->own) return;  // a view of a multiplane fitter's channel: mpCleanup frees it
+  h->own.reset();
   delete h;
 }
 
@@ -228,21 +234,22 @@ void mFitNewImage(fitData* fd, double* image) {
   run([&] {
     Handle* h = H(fd);
     h->pub.nfit = 0;
-    h->eng->new_image(0, image);
+    h->eng->new_image(h->frame, image);
   });
 }
 
 void mFitNewBackground(fitData* fd, double* background) {
-  run([&] { H(fd)->eng->new_background(0, background); });
+  run([&] { H(fd)->eng->new_background(H(fd)->frame, background); });
 }
 
 void mFitAnscombeTransformImage(fitData* fd) {
-  run([&] { H(fd)->eng->anscombe(0); });
+  run([&] { H(fd)->eng->anscombe(H(fd)->frame); });
 }
 
 void mFitIterateLM(fitData* fd) {
   run([&] {
     Handle* h = H(fd);
+    refuse_channel_view(h);
     if (h->eng->use_rounds)
       h->eng->sweep();
     else
@@ -259,6 +266,7 @@ void mFitIterateLM(fitData* fd) {
 int mFitDoFit(fitData* fd, int max_iterations) {
   return run_value(-1, [&] {
     Handle* h = H(fd);
+    refuse_channel_view(h);
     const int s = h->eng->use_rounds ? h->eng->fit(max_iterations) : h->eng->fit_frames(max_iterations);
     add_counters(h);
     check_frame(h);
@@ -267,19 +275,19 @@ int mFitDoFit(fitData* fd, int max_iterations) {
 }
 
 int mFitGetUnconverged(fitData* fd) {
-  return run_value(-1, [&] { return H(fd)->eng->count_status(0, dao::ST_RUNNING); });
+  return run_value(-1, [&] { return H(fd)->eng->count_status(H(fd)->frame, dao::ST_RUNNING); });
 }
 
 int mFitGetNError(fitData* fd) {
-  return run_value(-1, [&] { return H(fd)->eng->count_status(0, dao::ST_ERROR); });
+  return run_value(-1, [&] { return H(fd)->eng->count_status(H(fd)->frame, dao::ST_ERROR); });
 }
 
 void mFitGetFitImage(fitData* fd, double* out) {
-  run([&] { H(fd)->eng->get_image(0, out, 0); });
+  run([&] { H(fd)->eng->get_image(H(fd)->frame, out, 0); });
 }
 
 void mFitGetResidual(fitData* fd, double* out) {
-  run([&] { H(fd)->eng->get_image(0, out, 1); });
+  run([&] { H(fd)->eng->get_image(H(fd)->frame, out, 1); });
 }
 
 void mFitGetPeakPropertyDouble(fitData* fd, double* values, char* what) {
@@ -293,40 +301,40 @@ void mFitGetPeakPropertyDouble(fitData* fd, double* values, char* what) {
       for (int i = 0; i < n; i++) values[i] = v[i] + off;
     };
     if (!std::strcmp(what, "background")) {
-      e.get_param(0, dao::P_BG, v);
+      e.get_param(h->frame, dao::P_BG, v);
       put(0.0);
     } else if (!std::strcmp(what, "bg_sum")) {
-      e.get_sums(0, 0, v);
+      e.get_sums(h->frame, 0, v);
       put(0.0);
     } else if (!std::strcmp(what, "error")) {
-      e.get_error(0, v);
+      e.get_error(h->frame, v);
       put(0.0);
     } else if (!std::strcmp(what, "fg_sum") || !std::strcmp(what, "fg_sum_sc")) {
-      e.get_sums(0, 1, v);
+      e.get_sums(h->frame, 1, v);
       put(0.0);
     } else if (!std::strcmp(what, "height")) {
-      e.get_param(0, dao::P_HEIGHT, v);
+      e.get_param(h->frame, dao::P_HEIGHT, v);
       put(0.0);
     } else if (!std::strcmp(what, "jacobian")) {
-      e.get_jacobian(0, v);
+      e.get_jacobian(h->frame, v);
       for (size_t i = 0; i < v.size(); i++) values[i] = v[i];
     } else if (!std::strcmp(what, "sum")) {
-      e.get_sums(0, 2, v);
+      e.get_sums(h->frame, 2, v);
       put(0.0);
     } else if (!std::strcmp(what, "x")) {
-      e.get_param(0, dao::P_XC, v);
+      e.get_param(h->frame, dao::P_XC, v);
       put(p.xoff);
     } else if (!std::strcmp(what, "xsigma")) {
-      e.get_param(0, dao::P_XW, v);
+      e.get_param(h->frame, dao::P_XW, v);
       for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
     } else if (!std::strcmp(what, "y")) {
-      e.get_param(0, dao::P_YC, v);
+      e.get_param(h->frame, dao::P_YC, v);
       put(p.yoff);
     } else if (!std::strcmp(what, "ysigma")) {
-      e.get_param(0, dao::P_YW, v);
+      e.get_param(h->frame, dao::P_YW, v);
       for (int i = 0; i < n; i++) values[i] = std::sqrt(1.0 / (2.0 * v[i]));
     } else if (!std::strcmp(what, "z")) {
-      e.get_param(0, dao::P_ZC, v);
+      e.get_param(h->frame, dao::P_ZC, v);
       put(p.zoff);
     } else {
       std::printf("Unrecognized parameter '%s'!\n", what);
@@ -339,9 +347,9 @@ void mFitGetPeakPropertyInt(fitData* fd, int32_t* values, char* what) {
     Handle* h = H(fd);
     std::vector<int32_t> v;
     if (!std::strcmp(what, "iterations")) {
-      h->eng->get_iterations(0, v);
+      h->eng->get_iterations(h->frame, v);
     } else if (!std::strcmp(what, "status")) {
-      h->eng->get_status(0, v);
+      h->eng->get_status(h->frame, v);
     } else {
       std::printf("Unrecognized parameter '%s'!\n", what);
       return;
@@ -354,10 +362,10 @@ void mFitRemoveErrorPeaks(fitData* fd) {
   run([&] {
     Handle* h = H(fd);
     std::vector<int32_t> st, keep;
-    h->eng->get_status(0, st);
+    h->eng->get_status(h->frame, st);
     for (int i = 0; i < (int)st.size(); i++)
       if (st[i] != dao::ST_ERROR) keep.push_back(i);
-    h->eng->compact(0, keep);
+    h->eng->compact(h->frame, keep);
     h->pub.nfit = (int)keep.size();
   });
 }
@@ -366,7 +374,7 @@ void mFitRemoveRunningPeaks(fitData* fd) {
   run([&] {
     Handle* h = H(fd);
     std::vector<int32_t> st, keep;
-    h->eng->get_status(0, st);
+    h->eng->get_status(h->frame, st);
     int dropped = 0;
     for (int i = 0; i < (int)st.size(); i++) {
       if (st[i] != dao::ST_RUNNING)
@@ -374,14 +382,14 @@ void mFitRemoveRunningPeaks(fitData* fd) {
       else
         dropped++;
     }
-    h->eng->compact(0, keep);
+    h->eng->compact(h->frame, keep);
     h->pub.n_non_converged += dropped;
     h->pub.nfit = (int)keep.size();
   });
 }
 
 void mFitSetPeakStatus(fitData* fd, int32_t* status) {
-  run([&] { H(fd)->eng->set_status(0, status); });
+  run([&] { H(fd)->eng->set_status(H(fd)->frame, status); });
 }
 
 void daoSetTeam(fitData* fd, int peaks_per_workgroup) {

@@ -22,6 +22,8 @@ from storm_analysis_amd.sa_library import dao_fit_c  # noqa: E402
 # (test_oracle_mutations.py runs the pins against deliberately broken copies)
 ORACLE_SO = os.environ.get("DAO_ORACLE_SO", os.path.join(ROOT, "oracle", "liboracle_dao.so"))
 HIP_SO = os.path.join(ROOT, "storm-analysis_amd", "c_libraries", "libdao_fit.so")
+# the multiplane oracle (DAO_MP_ORACLE_SO: another build of it, as above)
+MP_ORACLE_SO = os.environ.get("DAO_MP_ORACLE_SO", os.path.join(ROOT, "oracle", "liboracle_mp.so"))
 
 
 def pytest_configure(config):
@@ -54,6 +56,29 @@ def make_fitter(cls, backend, **kwds):
     if backend == "oracle":
         lib = oracle_lib()
         with mock.patch.object(dao_fit_c, "loadDaoFitC", lambda: lib):
+            return cls(**kwds)
+    return cls(**kwds)
+
+
+_mp_oracle = None
+
+
+def mp_oracle_lib():
+    global _mp_oracle
+    if _mp_oracle is None:
+        if not os.path.exists(MP_ORACLE_SO):
+            raise RuntimeError("multiplane oracle not built: make -C oracle")
+        from storm_analysis_amd.multi_plane import mp_fit_dao_c
+        _mp_oracle = mp_fit_dao_c.setMPFitDaoArgtypes(ctypes.cdll.LoadLibrary(MP_ORACLE_SO))
+    return _mp_oracle
+
+
+def make_mp_fitter(cls, backend, **kwds):
+    """An instance of a multi_plane.mp_fit_dao_c class bound to 'oracle' or 'hip'."""
+    from storm_analysis_amd.multi_plane import mp_fit_dao_c
+    if backend == "oracle":
+        lib = mp_oracle_lib()
+        with mock.patch.object(mp_fit_dao_c, "loadMPFitCDao", lambda: lib):
             return cls(**kwds)
     return cls(**kwds)
 

@@ -40,10 +40,14 @@ def test_headers_declare_the_reference_surface():
               "mFitGetFitImage", "mFitGetResidual", "mFitGetPeakPropertyDouble", "mFitGetPeakPropertyInt",
               "mFitRemoveErrorPeaks", "mFitRemoveRunningPeaks", "mFitSetPeakStatus"):
         assert n in names, n
+    # the multiplane binding (storm_analysis/multi_plane/mp_fit_c.py:55-127, mp_fit_dao_c.py:17-36)
+    for n in ("mpCleanup", "mpInitialize", "mpIterateLM", "mpSetTransforms", "mpSetWeights",
+              "mpSetWeightsIndexing", "mpDaoInitialize2DChannel", "mpDaoNewPeaks"):
+        assert n in names, n
 
 
 # headers whose functions live in a library of their own (as in the reference)
-HEADER_LIBRARY = {"dao_mfilter_amd.h": "matched_filter"}
+HEADER_LIBRARY = {"dao_mfilter_amd.h": "matched_filter", "mp_fit_amd.h": "mp_fit_dao"}
 
 
 @pytest.mark.parametrize("header,name", declared())

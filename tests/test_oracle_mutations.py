@@ -58,19 +58,66 @@ MUTATIONS = [
 ]
 
 
+# The multiplane pins (test_mp_pins.py) against copies of oracle/mp_oracle.c
+MP_SRC = os.path.join(os.path.dirname(HERE), "oracle", "mp_oracle.c")
+MP_MUTATIONS = [
+    ("mp_heights_unweighted", "      sum += d * w * mp->heights[c];\n      norm += w * mp->heights[c];",
+     "      sum += d * w;\n      norm += w;", "test_first_coupled_step"),
+    ("mp_back_with_0toN", "const double *tr = axis == 0 ? mp->xt_Nto0 : mp->yt_Nto0;",
+     "const double *tr = axis == 0 ? mp->xt_0toN : mp->yt_0toN;", "test_first_coupled_step"),
+    ("mp_transposed_coefficient",
+     "double d = tr[3 * c + 1] * mp->w_jacobian[c][1];\n      d += tr[3 * c + 2] * mp->w_jacobian[c][2];",
+     "double d = tr[3 * c + 1] * mp->w_jacobian[c][2];\n      d += tr[3 * c + 2] * mp->w_jacobian[c][1];",
+     "test_first_coupled_step"),
+    ("mp_weights_channel_major", "  double w1 = wts[nc * zi + c];\n  double w2 = wts[nc * (zi + 1) + c];",
+     "  double w1 = wts[2 * c + zi];\n  double w2 = wts[2 * c + zi + 1];", "test_first_coupled_step"),
+    ("mp_y_width_follows", "    mp->fit_data[c]->working_peak->params[P_XW] -= mp->w_jacobian[c][3];\n",
+     "    mp->fit_data[c]->working_peak->params[P_XW] -= mp->w_jacobian[c][3];\n"
+     "    mp->fit_data[c]->working_peak->params[P_YW] = mp->fit_data[c]->working_peak->params[P_XW];\n",
+     "test_first_coupled_step"),
+    ("mp_map_without_offset", "    p->params[P_XC] = t - xoff;", "    p->params[P_XC] = t;",
+     "test_first_coupled_step"),
+    ("mp_step_offset_added", "double d = tr[3 * c + 1] * mp->w_jacobian[c][1];",
+     "double d = tr[3 * c] + tr[3 * c + 1] * mp->w_jacobian[c][1];", "test_a_whole_pixel_shifted_copy_changes_nothing"),
+    ("mp_mean_unnormalised", "    p0->params[axis == 0 ? P_XC : P_YC] -= sum / norm;",
+     "    p0->params[axis == 0 ? P_XC : P_YC] -= sum;", "test_a_whole_pixel_shifted_copy_changes_nothing"),
+    ("mp_no_height_floor", "    if (p->params[P_HEIGHT] < 0.01) p->params[P_HEIGHT] = 0.01;\n", "",
+     "test_heights_floor_at_a_hundredth"),
+    ("mp_decide_on_channel_0", "        else\n          cur += mp->fit_data[c]->working_peak->error;",
+     "        else if (c == 0)\n          cur += mp->fit_data[c]->working_peak->error;",
+     "test_the_summed_deviance_decides"),
+    ("mp_start_error_channel_0", "      start += fd->working_peak->error;",
+     "      if (c == 0) start += fd->working_peak->error;", "test_a_whole_pixel_shifted_copy_changes_nothing"),
+    ("mp_non_decr_channel_0", "          mp->fit_data[c]->n_non_decr++;",
+     "          if (c == 0) mp->fit_data[c]->n_non_decr++;", "test_the_summed_deviance_decides"),
+    ("mp_solve_every_channel", "      for (int c = 0; c < nc && !bad; c++) {", "      for (int c = 0; c < nc; c++) {",
+     "test_failed_solve_stops_the_channel_loop"),
+    ("mp_no_sync_at_new_peaks",
+     "  for (int i = (stop - n_peaks > 0 ? stop - n_peaks : 0); i < stop; i++) check_group(mp, i);",
+     "  (void)stop;", "test_a_group_off_one_channel_is_an_error_everywhere"),
+    ("mp_sync_old_range",
+     "  for (int i = (stop - n_peaks > 0 ? stop - n_peaks : 0); i < stop; i++) check_group(mp, i);",
+     "  for (int i = start; i < start + n_peaks && i < stop; i++) check_group(mp, i);",
+     "test_new_error_groups_after_storage_growth"),
+]
+
+
 def _build(tmp, name, text):
     src = os.path.join(tmp, name + ".c")
     with open(src, "w") as fh:
         fh.write(text)
     so = os.path.join(tmp, "lib" + name + ".so")
-    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-w", "-o", so, src, "-lm"], check=True)
+    # -I: the multiplane oracle includes dao_oracle.c from its own directory
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-w", "-I", os.path.dirname(SRC), "-o", so, src, "-lm"],
+                   check=True)
     return so
 
 
-def _pins(so, select=None):
+def _pins(so, select=None, tests="test_lm_branches.py", var="DAO_ORACLE_SO"):
     """Exit status of the pins against the oracle build `so` (None: it hung -- a failure too)."""
-    env = dict(os.environ, DAO_ORACLE_SO=so)
-    cmd = [sys.executable, "-m", "pytest", os.path.join(HERE, "test_lm_branches.py"), "-q", "-m", "not gpu",
+    env = dict(os.environ)
+    env[var] = so
+    cmd = [sys.executable, "-m", "pytest", os.path.join(HERE, tests), "-q", "-m", "not gpu",
            "-p", "no:cacheprovider"]
     if select:
         cmd += ["-k", select]
@@ -106,4 +153,23 @@ def test_planted_misreading_fails_its_pin(tmp_path, name, old, new, pin):
             "    if (fd->fit[i].status != ST_RUNNING) continue;\n    peak_copy(fd, &fd->fit[i], w);")
         assert "int i = fd->nfit - 1 - ii;" in text
     rc, out = _pins(_build(str(tmp_path), name, text), select=pin)
+    assert rc != 0, (name, out)   # failed, or hung
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_unmodified_multiplane_oracle_passes_the_pins(tmp_path):
+    with open(MP_SRC) as fh:
+        text = fh.read()
+    rc, out = _pins(_build(str(tmp_path), "mp_unmodified", text), tests="test_mp_pins.py", var="DAO_MP_ORACLE_SO")
+    assert rc == 0, out
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("name,old,new,pin", MP_MUTATIONS, ids=[m[0] for m in MP_MUTATIONS])
+def test_planted_multiplane_misreading_fails_its_pin(tmp_path, name, old, new, pin):
+    with open(MP_SRC) as fh:
+        text = fh.read()
+    assert text.count(old) == 1, name
+    rc, out = _pins(_build(str(tmp_path), name, text.replace(old, new)), select=pin, tests="test_mp_pins.py",
+                    var="DAO_MP_ORACLE_SO")
     assert rc != 0, (name, out)   # failed, or hung
