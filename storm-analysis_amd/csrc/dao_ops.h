@@ -173,15 +173,17 @@ struct MemDev {
   }
 };
 
-// Every earlier memory access of this wave has completed (a MemDev store is
-// then visible to every CU) before a later one is issued; the workgroup
-// fences keep the compiler from moving accesses across.  vmcnt(0) with
-// expcnt and lgkmcnt left open.
-__device__ inline void dev_order() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8));
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+// An agent-scope acquire-release fence: every memory access of this wave
+// before it happens before every access after it, for every workgroup of the
+// launch that synchronises with it.  Each publish of the cross-workgroup
+// protocols (op done, seat idle, generation, escape) is a relaxed MemDev
+// store after a dev_order(); each observation is a relaxed MemDev load
+// followed by a dev_order() -- release and acquire fences around relaxed
+// atomics, so the ordering follows from the HIP (C++) memory model alone.
+// On gfx950 it compiles to buffer_wbl2 sc1, s_waitcnt vmcnt(0) and
+// buffer_inv sc1: the writeback and invalidate cover cached (non-MemDev)
+// accesses too.
+__device__ inline void dev_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent"); }
 
 // ---------------------------------------------------------------------------
 // per-peak uniform state (every lane holds the same copy)
@@ -772,14 +774,16 @@ __device__ inline float mle_h(float u) {
 }
 
 // LDS slice of a mixed-precision wave, in doubles: x (FP64), rqe and scmos
-// (FP32), the shape table (64 FP64 (offset, exp) pairs), J/H (FP64).
+// (FP32), the shape table (64 FP64 (offset, exp) pairs), J/H (FP64), and the
+// turn-start fi - x of every ROI pixel (FP64, for eval_delta).
 template <int PPL>
 struct RoiLdsM {
   static constexpr int kRqe = PPL * kWave;
   static constexpr int kScm = kRqe + PPL * kWave / 2;
   static constexpr int kTab = kScm + PPL * kWave / 2;
   static constexpr int kHJ = kTab + 2 * kWave;
-  static constexpr int kDoubles = kHJ + 32;
+  static constexpr int kD0 = kHJ + 32;
+  static constexpr int kDoubles = kD0 + PPL * kWave;
 };
 
 template <int PPL>
@@ -795,6 +799,9 @@ struct RoiM {
   long long base;
   int at_x, at_y;
   bool valid;
+  // the turn-start fi - x (d0) is held for the ROI at (d0x, d0y)
+  int d0x, d0y;
+  bool d0ok;
   double* lds;
 
   __device__ bool ok(int s) const { return (okm >> s) & 1u; }
@@ -806,10 +813,12 @@ struct RoiM {
   __device__ float& scm(int s) { return reinterpret_cast<float*>(lds + RoiLdsM<PPL>::kScm)[s * kWave + lane_id()]; }
   __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLdsM<PPL>::kTab); }
   __device__ double* hj() { return lds + RoiLdsM<PPL>::kHJ; }
+  __device__ double& d0(int s) { return lds[RoiLdsM<PPL>::kD0 + s * kWave + lane_id()]; }
 
   __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
     int l = lane_id();
     okm = 0;
+    d0ok = false;
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
       int j = l + s * kWave;
@@ -949,6 +958,73 @@ __device__ int eval_error(const Model& m, RoiM<PPL>& c, const double* asx, doubl
     return 1;
   }
   err = wsum(sum);
+  return 0;
+}
+
+// fi - x of every held pixel in FP64, as eval_error forms it before
+// rounding to FP32.
+template <int PPL>
+__device__ inline double fi_minus_x(RoiM<PPL>& c, int s) {
+  const double cn = (double)c.cnt[s];
+  double rc = __builtin_amdgcn_rcp(cn);
+  rc = fma(fma(-cn, rc, 1.0), rc, rc);
+  return fma(c.bg[s], rc, c.f[s] - c.x(s));
+}
+
+// Keep the held pixels' fi - x as the turn's start (after a successful
+// eval_error of the start state).
+template <int PPL>
+__device__ void keep_start(RoiM<PPL>& c) {
+#pragma unroll
+  for (int s = 0; s < PPL; s++)
+    if (c.ok(s)) c.d0(s) = fi_minus_x(c, s);
+  c.d0x = c.at_x;
+  c.d0y = c.at_y;
+  c.d0ok = true;
+  wave_sync_lds();
+}
+
+// The change of the MLE deviance since keep_start(), over the same ROI: with
+// e = fi1 - fi0 (FP64) and t = e / fi0 (FP32), each pixel's change
+// 2 (fi1 - fi0) - 2 x ln(fi1 / fi0) is 2 (e (fi0 - x) / fi0 + x h(t)), h(t) =
+// t - ln(1 + t).  Summed, it carries the accuracy of FP32 relative to the
+// change itself -- not to the two totals, whose FP32 rounding is larger
+// than the 1e-6 relative tolerance the turn compares the change with.
+// Same fi <= 0 stop and cached values as eval_error.
+template <int PPL>
+__device__ int eval_delta(const Model& m, RoiM<PPL>& c, double& delta, int& neg_fi) {
+  const int l = lane_id();
+  double sum = 0.0;
+  bool stopped = false;
+#pragma unroll
+  for (int s = 0; s < PPL; s++) {
+    if (!stopped) {
+      const bool o = c.ok(s);
+      const double xk = o ? c.x(s) : 1.0;
+      const double dd1 = o ? fi_minus_x(c, s) : 0.0;
+      const float dd = (float)dd1;
+      const float xf = (float)xk;
+      const float fi = xf + dd;  // only for the fi <= 0 test
+      const unsigned long long bad = __ballot(o && fi <= 0.0f);
+      const int lim = bad ? __builtin_ctzll(bad) : kWave;
+      if (o && l < lim) {
+        const double d0 = c.d0(s);
+        const float e = (float)(dd1 - d0);
+        const float f0 = (float)(xk + d0);
+        const float r0 = __builtin_amdgcn_rcpf(f0);
+        const float t = e * r0;
+        sum += (double)(2.0f * (e * ((float)d0 * r0) + xf * mle_h(t)));
+        c.dfi[s] = dd;
+        c.stale &= ~(1u << s);
+      }
+      if (bad) stopped = true;
+    }
+  }
+  if (stopped) {
+    neg_fi += 1;
+    return 1;
+  }
+  delta = wsum(sum);
   return 0;
 }
 
@@ -1222,9 +1298,15 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   if (jr) nsnap = 1;
   c.shape(m, w);
 
+  // mixed precision judges a candidate at the start's ROI origin on the
+  // deviance change (eval_delta) instead of two rounded totals
+  constexpr bool kDelta = std::is_same<R, RoiM<PPL>>::value;
   double wJ[N];
   double e;
-  if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) w.err = e;
+  if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) {
+    w.err = e;
+    if constexpr (kDelta) keep_start<PPL>(c);
+  }
   const double start = w.err;
   calc_jh<PPL, SHAPE>(m, c, w, px.asx);
   const double* J = c.hj();
@@ -1280,7 +1362,22 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
     c.stamp(w, +1);
     w.added++;
 
-    if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) != 0) {
+    // dl = candidate error - start error (for FP64 exactly that difference)
+    double dl = 0.0;
+    int bad_e;
+    if constexpr (kDelta) {
+      if (c.d0ok && c.at_x == c.d0x && c.at_y == c.d0y) {
+        bad_e = eval_delta<PPL>(m, c, dl, d[C_NEG_FI]);
+        e = start + dl;
+      } else {
+        bad_e = eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]);
+        dl = e - start;
+      }
+    } else {
+      bad_e = eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]);
+      dl = e - start;
+    }
+    if (bad_e != 0) {
       c.stamp(w, -1);
       w.added--;
       reset_from<M>(w, rec);
@@ -1289,8 +1386,8 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
     }
     w.err = e;
 
-    if (w.err > start) {
-      if (((w.err - start) / start) < m.tol) {
+    if (dl > 0.0) {
+      if ((dl / start) < m.tol) {
         w.status = ST_CONVERGED;
         break;
       }
@@ -1299,7 +1396,7 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
       w.added--;
       reset_from<M>(w, rec);
     } else {
-      if (((start - w.err) / start) < m.tol) {
+      if (((-dl) / start) < m.tol) {
         w.status = ST_CONVERGED;
         break;
       }
@@ -1897,12 +1994,14 @@ struct TeamShared {
 constexpr int kActRun = 0, kActLead = 1, kActStop = 2;
 
 // The scheduler's cold paths (team_await, team_quiet, team_transition) are
-// kept out of line.  With them and the LM turn inlined into k_fit_team in
-// one register allocation, the fits came out wrong in two builds while every
-// build with them out of line matched the oracle, and instrumenting the
-// inlined build made the error vanish; the cause was not found.  Out of line
-// they also leave the hot loop (take, wait, turn) fewer registers to spill:
-// a team turn got ~12% faster than with the whole op loop out of line.
+// inlined into k_fit_team.  Measured on one MI355X with dev_order() as the
+// agent-scope fence above: a 2048^2 frame of 20,000 emitters with teams of
+// 500 peaks per workgroup fits in 499 ms inlined against 534 ms out of line
+// (-DDAO_TEAM_COLD=__noinline__), a 512^2 Z frame of 1000 emitters with
+// teams of 64 in 23.9 ms against 26.4 ms; both builds give identical fits.
+#ifndef DAO_TEAM_COLD
+#define DAO_TEAM_COLD inline
+#endif
 
 // Every wait of a team polls at most a.spin_limit times.  A wait that gives
 // up marks frame f failed (frame_err[f]); every wait of the frame's team
@@ -1916,7 +2015,7 @@ __device__ inline void team_fail(const FitArgs& a, int f) {
 // Wave 0: wait for a generation other than g, or claim the transition out
 // of g once it is due (g == 0: the frame's first run; then: the run's ops
 // are all done, or one escaped).  Returns the action; sh.gen its generation.
-__device__ __noinline__ int team_await(const FitArgs& a, int f, TeamCtl* ctl, int g, TeamShared& sh) {
+__device__ DAO_TEAM_COLD int team_await(const FitArgs& a, int f, TeamCtl* ctl, int g, TeamShared& sh) {
   for (long long spin = 0;; spin++) {
     if (team_failed(a, f)) return kActStop;
     const int ph = uniform_i(MemDev::ld(&ctl->phase));
@@ -1951,7 +2050,7 @@ __device__ __noinline__ int team_await(const FitArgs& a, int f, TeamCtl* ctl, in
 
 // Wave 0 of the claimant: wait until no seat of the frame holds or is
 // taking an op.  False if the frame failed.
-__device__ __noinline__ bool team_quiet(const FitArgs& a, int f, int s0, int s1) {
+__device__ DAO_TEAM_COLD bool team_quiet(const FitArgs& a, int f, int s0, int s1) {
   for (long long spin = 0;; spin++) {
     if (team_failed(a, f)) return false;
     bool busy = false;
@@ -2116,7 +2215,7 @@ __device__ void team_boxes_recenter(const FitArgs& a, const Frame& fr, int n) {
 // next run of doFit's loop (as k_fit_frames), or end the fit; then publish
 // generation g + 1.
 template <int PPL>
-__device__ __noinline__ void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int f, int g, TeamShared& sh,
+__device__ DAO_TEAM_COLD void team_transition(const FitArgs& a, const Frame& fr, TeamCtl* ctl, int f, int g, TeamShared& sh,
                                 double* slice) {
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   const int kind = MemDev::ld(&ctl->kind);
