@@ -130,8 +130,25 @@ int daoBatchGetFitImage(daoBatch *b, int frame, double *out);
  * mFitGetResidual multi_fit.c:880-888) of every frame, concatenated in frame
  * order at the frames' current sizes, into host or device memory. */
 int daoBatchGetFitImages(daoBatch *b, double *out, int residual, int on_device);
-/* mFitRemoveErrorPeaks for every frame. */
+/* mFitRemoveErrorPeaks (multi_fit.c:1704-1740) for every frame, on the device. */
 int daoBatchRemoveErrorPeaks(daoBatch *b);
+/* mFitRemoveRunningPeaks (multi_fit.c:1750-1779) for every frame, on the
+ * device; each removed peak counts in its frame's n_non_converged. */
+int daoBatchRemoveRunningPeaks(daoBatch *b);
+/* mFitSetPeakStatus (multi_fit.c:1809-1831) for every frame: status holds
+ * one entry per live peak, frames in order (the rows of
+ * daoBatchGetAllPeaks); a peak that becomes ERROR while in the fit image is
+ * subtracted.  status is a host array, or a device array with on_device. */
+int daoBatchSetPeakStatus(daoBatch *b, const int32_t *status, int on_device);
+/* The "significance" property (fg_sum / sqrt(bg_sum), dao_fit_c.py:398-439,
+ * multi_fit.c:1518-1606) of every live peak, frames in order, into a
+ * device array.  No counterpart in the reference's C library. */
+int daoBatchGetSignificanceDevice(daoBatch *b, double *out);
+/* daoBatchFit limited to the frames whose active[f] (host, one per frame)
+ * is non-zero; the others are left as they are.  No counterpart in the
+ * reference: a batch runs the reference's per-frame find / fit loop
+ * (fitting.py:866-902), in which only some frames fit at each step. */
+int daoBatchFitActive(daoBatch *b, int max_iterations, const int32_t *active);
 
 /* Instrumentation: enable HIP-event timing of the LM kernel; stats out[6] =
  * {LM kernel ms, LM launches, sweeps, executor rounds, escape re-runs, max attempts}. */

@@ -452,16 +452,47 @@ int daoBatchGetFitImages(daoBatch* b, double* out, int residual, int on_device) 
 }
 
 int daoBatchRemoveErrorPeaks(daoBatch* b) {
+  return dao::guarded([&] { b->eng->remove_all(dao::ST_ERROR); });
+}
+
+int daoBatchRemoveRunningPeaks(daoBatch* b) {
+  return dao::guarded([&] {
+    b->eng->remove_all(dao::ST_RUNNING);
+    pull_counters(b);
+  });
+}
+
+int daoBatchSetPeakStatus(daoBatch* b, const int32_t* status, int on_device) {
   return dao::guarded([&] {
     Engine& e = *b->eng;
-    for (int f = 0; f < e.F; f++) {
-      std::vector<int32_t> st, keep;
-      e.get_status(f, st);
-      for (int i = 0; i < (int)st.size(); i++)
-        if (st[i] != dao::ST_ERROR) keep.push_back(i);
-      e.compact(f, keep);
+    const long long n = e.total_nfit();
+    if (n == 0) return;
+    if (status == nullptr) dao::fail(dao::kErrArg, "status is NULL");
+    if (on_device) {
+      e.set_status_all(status);
+      return;
     }
+    int32_t* d = reinterpret_cast<int32_t*>(tmp(b, sizeof(int32_t) * (size_t)n));
+    DAO_CHECK(hipMemcpy(d, status, sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice));
+    e.set_status_all(d);
   });
+}
+
+int daoBatchGetSignificanceDevice(daoBatch* b, double* out) {
+  return dao::guarded([&] {
+    if (b->eng->total_nfit() > 0 && out == nullptr) dao::fail(dao::kErrArg, "out is NULL");
+    b->eng->significance_all(out);
+  });
+}
+
+int daoBatchFitActive(daoBatch* b, int max_iterations, const int32_t* active) {
+  Engine& e = *b->eng;
+  if (active == nullptr) return daoBatchFit(b, max_iterations);
+  e.fit_mask.assign(e.F, 0);
+  for (int f = 0; f < e.F; f++) e.fit_mask[f] = active[f] != 0;
+  const int rc = daoBatchFit(b, max_iterations);
+  e.fit_mask.clear();
+  return rc;
 }
 
 void daoBatchSetTiming(daoBatch* b, int on) { b->eng->timing = on != 0; }

@@ -493,7 +493,7 @@ int Engine::team_blocks(std::vector<int>& tab, int& nseats) const {
 int Engine::fit_frames(int max_more) {
   std::vector<int> list;
   for (int f = 0; f < F; f++)
-    if (fr[f].nfit > 0) list.push_back(f);
+    if (fr[f].nfit > 0 && (fit_mask.empty() || fit_mask[f])) list.push_back(f);
   if (list.empty()) return 0;
   if (d_sweeps == nullptr) {
     d_sweeps = dalloc<int>(F);
@@ -593,6 +593,7 @@ int Engine::fit_frames(int max_more) {
 
 int Engine::fit(int max_iterations) {
   if (!use_rounds) return fit_frames(max_iterations);
+  if (!fit_mask.empty()) fail(kErrArg, "a frame mask needs the persistent fitter (not the rounds executor)");
   int sweeps = 1;
   sweep();
   int i = 0;
@@ -727,6 +728,64 @@ void Engine::compact(int f, const std::vector<int32_t>& keep) {
   }
   fr[f].nfit = n;
   upload_frames();
+  sync();
+}
+
+// The copy is waited for: the pinned buffer is reused by the next read-back.
+int* Engine::upload_offsets(std::vector<int>& off) {
+  off.assign(F + 1, 0);
+  for (int f = 0; f < F; f++) off[f + 1] = off[f] + fr[f].nfit;
+  int* d = istage(sizeof(int) * (size_t)(F + 1));
+  int* h = pin(sizeof(int) * (size_t)(F + 1));
+  std::memcpy(h, off.data(), sizeof(int) * (size_t)(F + 1));
+  DAO_CHECK(hipMemcpyAsync(d, h, sizeof(int) * (size_t)(F + 1), hipMemcpyHostToDevice, st));
+  sync();
+  return d;
+}
+
+void Engine::set_status_all(const int32_t* status_dev) {
+  if (total_nfit() == 0) return;
+  std::vector<int> off;
+  int* d_off = upload_offsets(off);
+  // peaks leaving the fit image: subtracted by the executor, in slot order
+  launch_status_flag(pk, d_fr, d_off, cap, status_dev, d_flag, st);
+  launch_compact_list(d_flag, cap, sc.op_peak, d_count, st);
+  int* h = pin(sizeof(int));
+  DAO_CHECK(hipMemcpyAsync(h, d_count, sizeof(int), hipMemcpyDeviceToHost, st));
+  sync();
+  const int nops = h[0];
+  if (nops > 0) {
+    sc.nops = nops;
+    launch_fill_i32(sc.frame_bad, F, 0, st);
+    launch_box(OP_SUBTRACT, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
+    run_ops(OP_SUBTRACT, nullptr, 0, 0);
+  }
+  // d_off may have been reused by the executor: upload again
+  d_off = upload_offsets(off);
+  launch_apply_status(pk, d_fr, d_off, cap, status_dev, st);
+  sync();
+}
+
+void Engine::remove_all(int drop) {
+  if (total_nfit() == 0) return;
+  int* d_counts = istage(sizeof(int) * (size_t)F);
+  launch_compact_frames(pk, d_fr, F, drop, d_counts, d_counters, st);
+  int* h = pin(sizeof(int) * (size_t)F);
+  DAO_CHECK(hipMemcpyAsync(h, d_counts, sizeof(int) * (size_t)F, hipMemcpyDeviceToHost, st));
+  sync();
+  for (int f = 0; f < F; f++) fr[f].nfit = h[f];
+  upload_frames();
+  sync();
+}
+
+void Engine::significance_all(double* out_dev) {
+  const long long total = total_nfit();
+  if (total == 0) return;
+  std::vector<int> off;
+  int* d_off = upload_offsets(off);
+  int* d_slots = sc.op_peak;  // a slot list of `cap` entries
+  launch_packed_slots(d_fr, F, d_off, d_slots, st);
+  launch_peak_sums(m, px, pk, d_fr, d_roi, 0, (int)total, 3, out_dev, st, d_slots);
   sync();
 }
 

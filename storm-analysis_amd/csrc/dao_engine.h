@@ -86,7 +86,16 @@ void launch_set_calibration_const(const Pix& px, long long pix0, long long n, do
 void launch_anscombe(const Pix& px, long long pix0, long long n, hipStream_t st);
 void launch_fit_image(const Pix& px, long long pix0, long long n, double* out, int residual, hipStream_t st);
 void launch_peak_sums(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames, const int* roi_tab,
-                      int slot0, int n, int what, double* out, hipStream_t st);
+                      int slot0, int n, int what, double* out, hipStream_t st, const int* slots = nullptr);
+void launch_packed_slots(const Frame* frames, int nframes, const int* off, int* slots, hipStream_t st);
+void launch_status_flag(const Peaks& pk, const Frame* frames, const int* off, long long nslots, const int* status,
+                        int* flag, hipStream_t st);
+void launch_apply_status(const Peaks& pk, const Frame* frames, const int* off, long long nslots, const int* status,
+                         hipStream_t st);
+// order-preserving list of the flagged entries of flag[0, n) into out, count in *count
+void launch_compact_list(const int* flag, long long n, int* out, int* count, hipStream_t st);
+void launch_compact_frames(const Peaks& pk, const Frame* frames, int nframes, int drop, int* counts, int* counters,
+                           hipStream_t st);
 void launch_peak_jacobian(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames,
                           const int* roi_tab, int slot0, int n, double* out, hipStream_t st);
 void launch_gather_peaks(PeakRec* dst, const PeakRec* src, const int* map, int slot0, int n, hipStream_t st);
@@ -170,6 +179,16 @@ class Engine {
                        const std::vector<int>& counts, int fmt);
   void compact(int f, const std::vector<int32_t>& keep);
   void set_status(int f, const int32_t* status);
+  // every frame at once, on the device (rows in frame order, as pack_all):
+  // mFitSetPeakStatus from a device status array; mFitRemoveErrorPeaks
+  // (drop ST_ERROR) / mFitRemoveRunningPeaks (drop ST_RUNNING); the peaks'
+  // significance into a device array
+  void set_status_all(const int32_t* status_dev);
+  void remove_all(int drop);
+  void significance_all(double* out_dev);
+  // frames the next fit may touch (empty: all); set by the caller, cleared
+  // by nothing
+  std::vector<char> fit_mask;
   // fitting
   // one mFitIterateLM over every frame of the batch; returns the number of
   // RUNNING peaks it started with (0: nothing to do)
@@ -233,6 +252,8 @@ class Engine {
   int* pin(size_t bytes);
   double* stage(size_t bytes);
   int* istage(size_t bytes);
+  // row offsets of the frames (frame order, F + 1 entries) on the device
+  int* upload_offsets(std::vector<int>& off);
   int select(int mode);
   bool packed() const;  // every frame at its creation size
   void run_ops(int kind, const double* rows, int fmt, int first_slot);

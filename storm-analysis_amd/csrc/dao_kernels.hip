@@ -355,14 +355,16 @@ __global__ void k_collect_escapes(Sched sc, const Peaks pk, int4* override_box) 
 }
 
 // Per-peak derived properties: bg_sum (what 0), fg_sum (1), sum (2)
-// (multi_fit.c:1518-1632).
+// (multi_fit.c:1518-1632), significance fg_sum / sqrt(bg_sum) (3; the
+// "compound" property of dao_fit_c.py:398-439).  Peak i is slot slot0 + i,
+// or slots[i] when a slot list is given.
 template <int PPL>
 __global__ __launch_bounds__(256) void k_peak_sums(Model m, Pix px, Peaks pk, const Frame* frames,
-                                                   const int* roi_tab, int slot0, int n, int what,
+                                                   const int* roi_tab, int slot0, const int* slots, int n, int what,
                                                    double* out) {
   const int i = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (i >= n) return;
-  const int p = slot0 + i;
+  const int p = slots ? slots[i] : slot0 + i;
   PeakState w;
   load_state(pk.rec + p, w);
   const Frame fr = frames[pk.rec[p].frame];
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(256) void k_peak_sums(Model m, Pix px, Peaks pk, co
   c.init(m, roi_tab, fr, nullptr);
   double t, e;
   lane_shape(m, w, t, e);
-  double ps = 0.0, a = 0.0;
+  double ps = 0.0, a = 0.0, a2 = 0.0;
 #pragma unroll
   for (int s = 0; s < PPL; s++) {
     double ex = __shfl(e, c.rx(s), kWave);
@@ -379,28 +381,103 @@ __global__ __launch_bounds__(256) void k_peak_sums(Model m, Pix px, Peaks pk, co
       double psf = ey * ex;
       ps += psf;
       int gx = w.xi + c.rx(s), gy = w.yi + c.ry(s);
-      if (what == 0 && gx >= 0 && gy >= 0 && gx < fr.W && gy < fr.H) {
+      if ((what == 0 || what == 3) && gx >= 0 && gy >= 0 && gx < fr.W && gy < fr.H) {
         const PixRO r = px.ro[fr.pix0 + (long long)gy * fr.W + gx];
         double b = (r.x - r.scm) / r.rqe - w.prm[P_HEIGHT] * psf;
         a += b * psf * psf;
-      } else if (what == 1) {
-        a += psf * psf;
       }
+      if (what == 1 || what == 3) a2 += psf * psf;
     }
   }
   ps = wsum(ps);
   a = wsum(a);
+  a2 = wsum(a2);
   if (lane_id() == 0) {
+    const double bg_sum = ((a > 0.0) && (ps > 0.0)) ? a / (ps * ps) : 1.0;
+    const double fg_sum = ps > 0.0 ? (w.prm[P_HEIGHT] * a2) / ps : 1.0;
     double v;
     if (what == 0) {
-      v = ((a > 0.0) && (ps > 0.0)) ? a / (ps * ps) : 1.0;
+      v = bg_sum;
     } else if (what == 1) {
-      a = w.prm[P_HEIGHT] * a;
-      v = ps > 0.0 ? a / ps : 1.0;
+      v = fg_sum;
+    } else if (what == 3) {
+      v = fg_sum / sqrt(bg_sum);
     } else {
       v = ps * w.prm[P_HEIGHT];
     }
     out[i] = v;
+  }
+}
+
+// The slots of every frame's live peaks in frame order (off[f] = the
+// frame's first row): one block per frame.
+__global__ void k_packed_slots(const Frame* frames, const int* off, int* slots) {
+  const Frame fr = frames[blockIdx.x];
+  for (int i = threadIdx.x; i < fr.nfit; i += blockDim.x) slots[off[blockIdx.x] + i] = fr.peak0 + i;
+}
+
+// mFitSetPeakStatus (multi_fit.c:1809-1831) for every frame: flag[slot]
+// marks a live peak that the new status (rows in frame order) makes ERROR
+// while it is in the fit image (it must be subtracted first).
+__global__ void k_status_flag(Peaks pk, const Frame* frames, const int* off, long long nslots, const int* status,
+                              int* flag) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const PeakRec r = pk.rec[t];
+  int sel = 0;
+  if (r.frame >= 0) {
+    const Frame fr = frames[r.frame];
+    const long long j = t - fr.peak0;
+    if (j >= 0 && j < fr.nfit) sel = (status[off[r.frame] + j] == ST_ERROR && r.added > 0) ? 1 : 0;
+  }
+  flag[t] = sel;
+}
+
+__global__ void k_apply_status(Peaks pk, const Frame* frames, const int* off, long long nslots, const int* status) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nslots) return;
+  const int fi = pk.rec[t].frame;
+  if (fi < 0) return;
+  const Frame fr = frames[fi];
+  const long long j = t - fr.peak0;
+  if (j >= 0 && j < fr.nfit) pk.rec[t].status = status[off[fi] + j];
+}
+
+// mFitRemoveErrorPeaks / mFitRemoveRunningPeaks (multi_fit.c:1704-1779) for
+// every frame at once: one block per frame keeps, in order, the live peaks
+// whose status is not `drop` (gathered through `bk`, then copied back) and
+// writes the frame's new count; removed RUNNING peaks count as
+// n_non_converged.
+__global__ __launch_bounds__(1024) void k_compact_frames(Peaks pk, const Frame* frames, int drop, int* counts,
+                                                         int* counters) {
+  __shared__ int part[1024];
+  const int f = blockIdx.x;
+  const Frame fr = frames[f];
+  const int n = fr.nfit;
+  const int t = threadIdx.x;
+  const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int a = min(t * per, n), b = min(a + per, n);
+  int mine = 0;
+  for (int i = a; i < b; i++) mine += pk.rec[fr.peak0 + i].status != drop;
+  part[t] = mine;
+  __syncthreads();
+  for (int d = 1; d < (int)blockDim.x; d <<= 1) {
+    const int v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int at = part[t] - mine;
+  for (int i = a; i < b; i++) {
+    const PeakRec r = pk.rec[fr.peak0 + i];
+    if (r.status != drop) pk.bk[fr.peak0 + at++] = r;
+  }
+  const int kept = part[blockDim.x - 1];
+  __syncthreads();
+  for (int i = t; i < kept; i += blockDim.x) pk.rec[fr.peak0 + i] = pk.bk[fr.peak0 + i];
+  if (t == 0) {
+    counts[f] = kept;
+    if (drop == ST_RUNNING && n > kept) atomicAdd(&counters[f * C_COUNT + C_NON_CONV], n - kept);
   }
 }
 
@@ -617,10 +694,28 @@ void launch_fit_image(const Pix& px, long long pix0, long long n, double* out, i
   k_fit_image<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, out, residual);
 }
 void launch_peak_sums(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames, const int* roi_tab,
-                      int slot0, int n, int what, double* out, hipStream_t st) {
+                      int slot0, int n, int what, double* out, hipStream_t st, const int* slots) {
   if (n == 0) return;
   DAO_PPL_SWITCH(m.nroi, (k_peak_sums<PPL><<<waves_grid(n), 256, 0, st>>>(m, px, pk, frames, roi_tab, slot0,
-                                                                         n, what, out)));
+                                                                         slots, n, what, out)));
+}
+void launch_packed_slots(const Frame* frames, int nframes, const int* off, int* slots, hipStream_t st) {
+  if (nframes > 0) k_packed_slots<<<nframes, 256, 0, st>>>(frames, off, slots);
+}
+void launch_status_flag(const Peaks& pk, const Frame* frames, const int* off, long long nslots, const int* status,
+                        int* flag, hipStream_t st) {
+  if (nslots > 0) k_status_flag<<<flat_grid(nslots), 256, 0, st>>>(pk, frames, off, nslots, status, flag);
+}
+void launch_apply_status(const Peaks& pk, const Frame* frames, const int* off, long long nslots, const int* status,
+                         hipStream_t st) {
+  if (nslots > 0) k_apply_status<<<flat_grid(nslots), 256, 0, st>>>(pk, frames, off, nslots, status);
+}
+void launch_compact_list(const int* flag, long long n, int* out, int* count, hipStream_t st) {
+  k_compact<<<1, 1024, 0, st>>>(flag, n, out, count);
+}
+void launch_compact_frames(const Peaks& pk, const Frame* frames, int nframes, int drop, int* counts, int* counters,
+                           hipStream_t st) {
+  if (nframes > 0) k_compact_frames<<<nframes, 1024, 0, st>>>(pk, frames, drop, counts, counters);
 }
 void launch_peak_jacobian(const Model& m, const Pix& px, const Peaks& pk, const Frame* frames,
                           const int* roi_tab, int slot0, int n, double* out, hipStream_t st) {

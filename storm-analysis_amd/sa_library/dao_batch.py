@@ -55,11 +55,12 @@ _BATCH_ENTRY_POINTS = (
     "daoBatchCreate", "daoBatchSetCalibration", "daoBatchNewImages", "daoBatchNewBackgrounds",
     "daoBatchAnscombe", "daoBatchNewPeaks", "daoBatchFit", "daoBatchGetPeaks", "daoBatchGetAllPeaks",
     "daoBatchGetAllPeaksDevice", "daoBatchGetFitImage", "daoBatchGetFitImages", "daoBatchRemoveErrorPeaks",
-    "daoBatchLoadFrame", "daoBatchClearFrame", "daoBatchLoadFrames", "daoBatchClearFrames")
+    "daoBatchLoadFrame", "daoBatchClearFrame", "daoBatchLoadFrames", "daoBatchClearFrames",
+    "daoBatchRemoveRunningPeaks", "daoBatchSetPeakStatus", "daoBatchGetSignificanceDevice", "daoBatchFitActive")
 
 
 def _batch_error(code, msg, func, args):
-    if code == _FRAME_FAILED and func.__name__ == "daoBatchFit":
+    if code == _FRAME_FAILED and func.__name__ in ("daoBatchFit", "daoBatchFitActive"):
         lib = loadBatchC()
         n = lib.daoBatchNFrames(args[0])
         flags = numpy.zeros(max(n, 1), dtype=numpy.int32)
@@ -102,6 +103,11 @@ def loadBatchC():
         lib.daoBatchGetFitImage.argtypes = [vp, ctypes.c_int, f64p]
         lib.daoBatchGetFitImages.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
         lib.daoBatchRemoveErrorPeaks.argtypes = [vp]
+        lib.daoBatchRemoveRunningPeaks.argtypes = [vp]
+        lib.daoBatchSetPeakStatus.argtypes = [vp, vp, ctypes.c_int]
+        lib.daoBatchGetSignificanceDevice.argtypes = [vp, vp]
+        lib.daoBatchFitActive.argtypes = [vp, ctypes.c_int, vp]
+        lib.daoBatchFitActive.restype = ctypes.c_int
         lib.daoBatchLoadFrame.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp,
                                           ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         lib.daoBatchLoadFrame.restype = ctypes.c_int
@@ -339,8 +345,50 @@ class BatchFitter(object):
         """Polls after which a scheduler wait gives up and fails its frame (0: default)."""
         self.clib.daoBatchSetWaitLimit(self.handle, int(polls))
 
+    def doFitFrames(self, active, max_iterations=2000):
+        """
+        doFit limited to the frames whose active[f] is true (the others are
+        left as they are); returns the sweeps run.
+        """
+        act = numpy.ascontiguousarray(numpy.asarray(active, dtype=bool).astype(numpy.int32))
+        if act.shape != (self.n_frames,):
+            raise MultiFitterException("active must have one entry per frame (%d)" % self.n_frames)
+        return self.clib.daoBatchFitActive(self.handle, max_iterations, act.ctypes.data_as(ctypes.c_void_p))
+
     def removeErrorPeaks(self):
         self.clib.daoBatchRemoveErrorPeaks(self.handle)
+
+    def removeRunningPeaks(self):
+        """mFitRemoveRunningPeaks of every frame (each removal counts as n_non_converged)."""
+        self.clib.daoBatchRemoveRunningPeaks(self.handle)
+
+    def setPeakStatus(self, status):
+        """
+        mFitSetPeakStatus of every frame: one int32 status per live peak, frames
+        in order (the rows of getAllPeaks), a numpy array or a torch CUDA tensor.
+        """
+        n = int(sum(self.getNFit(f) for f in range(self.n_frames)))
+        if hasattr(status, "is_cuda") and status.is_cuda:
+            import torch
+            if status.dtype != torch.int32 or not status.is_contiguous() or status.numel() != n:
+                raise MultiFitterException("status must be a contiguous int32 tensor of %d entries" % n)
+            torch.cuda.current_stream(status.device).synchronize()
+            self.clib.daoBatchSetPeakStatus(self.handle, ctypes.c_void_p(status.data_ptr()), 1)
+            return
+        st = numpy.ascontiguousarray(status, dtype=numpy.int32)
+        if st.size != n:
+            raise MultiFitterException("status must have %d entries" % n)
+        self.clib.daoBatchSetPeakStatus(self.handle, st.ctypes.data_as(ctypes.c_void_p), 0)
+
+    def getSignificanceDevice(self, device):
+        """The peaks' significance (fg_sum / sqrt(bg_sum)), frames in order, as a torch tensor on `device`."""
+        import torch
+        n = int(sum(self.getNFit(f) for f in range(self.n_frames)))
+        out = torch.empty(n, dtype=torch.float64, device=device)
+        if n:
+            torch.cuda.current_stream(out.device).synchronize()
+            self.clib.daoBatchGetSignificanceDevice(self.handle, ctypes.c_void_p(out.data_ptr()))
+        return out
 
     def getNFit(self, frame):
         return self.clib.daoBatchNFit(self.handle, frame)
