@@ -51,10 +51,10 @@ ALG_BYTES_PER_PIXEL = 36
 # --copy-calibration 1024`: FETCH_SIZE reads half the bytes moved, WRITE_SIZE
 # all of them).
 MEASURED_TRAFFIC = {
-    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0,
-              "fetch_kib": 95110555.8, "write_kib": 102390141.1, "fetch_scale": 2.0, "write_scale": 1.0},
-    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0,
-             "fetch_kib": 116141474.5, "write_kib": 140691942.9, "fetch_scale": 2.0, "write_scale": 1.0},
+    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "b72197a",
+              "fetch_kib": 99762901.25, "write_kib": 101853558.1875, "fetch_scale": 2.0, "write_scale": 1.0},
+    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "b72197a",
+             "fetch_kib": 113841120.75, "write_kib": 134037673.53125, "fetch_scale": 2.0, "write_scale": 1.0},
 }
 # accuracy bar of mixed precision against FP64 for x and y
 TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
@@ -285,6 +285,7 @@ def run_batched(precision, args, images, backgrounds, rows, counts_np, host=None
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 --pmc, copy-scaled)",
+                     "traffic_commit": mt.get("commit") if traffic is not None else None,
                      "alg_bytes_per_launch": alg_bytes / max(1, launches),
                      "kernel": "k_fit_frames (persistent frame-per-workgroup doFit, %s)" % precision,
                      "lm_ms": lm_ms, "lm_launches": launches,
