@@ -90,6 +90,49 @@ int mpDoFit(mpFit *mp, int max_iterations);
 /* mpCleanup (mp_fit.c:73-108). */
 void mpCleanup(mpFit *mp);
 
+/*
+ * Batched multiplane fitting.  No counterpart in the reference, which fits
+ * one frame-set (a frame of every channel) at a time: here n_sets frame-sets
+ * of one size share one device engine and one launch fits them all, one
+ * wavefront per frame-set, with mpIterateLM's arithmetic (the same results
+ * as the single-set handle).  Channel c of set s is engine frame s * C + c;
+ * arrays that hold every frame are in that order.  Calls return 0 (mpBatchFit:
+ * the largest sweep count of a set) or a negative code with
+ * daoGetLastError() set.
+ */
+typedef struct mpBatch mpBatch;
+/* 2D Gaussian MLE fitters, ROI roi_size, tolerance as mpInitialize's (the
+ * single-channel one times the channel count), peak_cap groups per set. */
+mpBatch *mpBatchCreate(int n_sets, int n_channels, int im_size_x, int im_size_y, int roi_size, double tolerance,
+                       int peak_cap);
+void mpBatchDestroy(mpBatch *b);
+/* mpDaoInitialize2DChannel for channel `channel` of every set. */
+int mpBatchInitializeChannel(mpBatch *b, int channel, const double *rqe, const double *variance, double width_min,
+                             double width_max);
+/* mpSetTransforms. */
+int mpBatchSetTransforms(mpBatch *b, const double *xt_0toN, const double *yt_0toN, const double *xt_Nto0,
+                         const double *yt_Nto0);
+/* The x / y weights of mpSetWeights (z_size x n_channels each) and mpSetWeightsIndexing. */
+int mpBatchSetWeights(mpBatch *b, const double *w_x, const double *w_y, int z_size, double z_offset, double z_scale);
+/* mFitNewImage / mFitNewBackground of every frame (host, or device with on_device). */
+int mpBatchNewImages(mpBatch *b, const double *images, int on_device);
+int mpBatchNewBackgrounds(mpBatch *b, const double *backgrounds, int on_device);
+/* mpDaoNewPeaks of every set: counts[s] channel-0 rows for set s, sets in order. */
+int mpBatchNewPeaks(mpBatch *b, const double *rows, const int *counts, const char *p_type, int on_device);
+/* MultiFitter.doFit driving mpIterateLM, for every set. */
+int mpBatchFit(mpBatch *b, int max_iterations);
+/* Groups held by set `set` (-1 if out of range). */
+int mpBatchNFit(mpBatch *b, int set);
+/* Every frame's peaks in frame order: rows of 9 doubles (x, y, z, height,
+ * background, xsigma, ysigma, error, iterations) and status. */
+int mpBatchGetAllPeaks(mpBatch *b, double *out, int32_t *status, int on_device);
+/* mFitRemoveErrorPeaks / mFitRemoveRunningPeaks of every frame. */
+int mpBatchRemoveErrorPeaks(mpBatch *b);
+int mpBatchRemoveRunningPeaks(mpBatch *b);
+/* Counters of one channel of one set since the last call (fitData order, as
+ * daoBatchCounters). */
+void mpBatchCounters(mpBatch *b, int set, int channel, int32_t *out8);
+
 #ifdef __cplusplus
 }
 #endif
