@@ -51,11 +51,12 @@ __device__ inline int4 uniform_i4(int4 v) {
 }
 
 // v of the lane selected by the DPP control CTRL (quad permutes and row
-// mirrors: every lane has a source inside its 16-lane row).
+// mirrors: every lane has a source inside its 16-lane row, so no lane keeps
+// an old value and none needs to be supplied).
 template <int CTRL>
 __device__ inline double dpp_mov(double v) {
-  int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
-  int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xf, 0xf, false);
+  int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
 }
 
@@ -85,6 +86,66 @@ __device__ inline double wsum(double v) {
   v += dpp_mov<0x141>(v);  // row_half_mirror
   v += dpp_mov<0x140>(v);  // row_mirror
   return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// Sums over the wave of K values per lane (v[k] of every lane), each summed
+// in wsum's order, into out[0, K) (LDS, read back as uniform values).  A
+// transposed butterfly: at the pair level (lane bit 0) each lane keeps half
+// of the values and adds its partner's copy of that half; at the quad level
+// (bit 1) half again; the K/4 partial sums of each lane then go to `red`
+// (64 * kRedSlots doubles of the wave's LDS slice), where lane k < K adds
+// value k's quads and rows in wsum's order.  Every sum is formed from the
+// same operands in the same tree as wsum's, so the results are wsum's bit for
+// bit, with about a quarter of the cross-lane moves.
+constexpr int kRedSlots = 7;  // ceil(ceil(K / 2) / 2) for K <= 28
+
+template <class T, int K>
+__device__ inline T pick(const T (&v)[K], int k) {
+  return k < K ? v[k] : T(0);
+}
+
+template <int K>
+struct Fold {
+  static constexpr int K1 = (K + 1) & ~1;
+  static constexpr int H1 = K1 / 2;          // values per lane after the pair level
+  static constexpr int H2 = (H1 + 1) / 2;    // after the quad level
+};
+
+template <int CTRL, int K, int H>
+__device__ inline void fold_level(const double (&v)[K], double (&out)[H], bool upper) {
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    const double lo = pick(v, i), hi = pick(v, i + H);
+    const double keep = upper ? hi : lo, send = upper ? lo : hi;
+    out[i] = keep + dpp_mov<CTRL>(send);
+  }
+}
+
+template <int K>
+__device__ void wsum_many(const double (&v)[K], double* red, double* out) {
+  using F = Fold<K>;
+  static_assert(F::H2 <= kRedSlots, "reduction slots");
+  const int l = lane_id();
+  double a[F::H1];
+  fold_level<0xB1>(v, a, (l & 1) != 0);  // quad_perm [1,0,3,2]: lane bit 0
+  double b[F::H2];
+  fold_level<0x4E>(a, b, (l & 2) != 0);  // quad_perm [2,3,0,1]: lane bit 1
+#pragma unroll
+  for (int j = 0; j < F::H2; j++) red[j * kWave + l] = b[j];
+  wave_sync_lds();
+  if (l < K) {
+    const int i = l % F::H1;
+    const int lane0 = ((i / F::H2) << 1) | (l / F::H1);  // holder of value l in quad 0 of row 0
+    const double* q = red + (i % F::H2) * kWave + lane0;
+    double r[4];
+#pragma unroll
+    for (int row = 0; row < 4; row++) {
+      const double* p = q + row * 16;
+      r[row] = (p[0] + p[4]) + (p[8] + p[12]);
+    }
+    out[l] = (r[0] + r[1]) + (r[2] + r[3]);
+  }
+  wave_sync_lds();
 }
 
 // ---------------------------------------------------------------------------
@@ -417,7 +478,8 @@ template <int PPL>
 struct RoiLds {
   static constexpr int kTab = 3 * PPL * kWave;
   static constexpr int kHJ = kTab + 2 * kWave;
-  static constexpr int kDoubles = kHJ + 32;
+  static constexpr int kRed = kHJ + 32;  // wsum_many's partial sums
+  static constexpr int kDoubles = kRed + kRedSlots * kWave;
 };
 
 template <int PPL>
@@ -519,6 +581,7 @@ struct Roi {
 
   __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLds<PPL>::kTab); }
   __device__ double* hj() { return lds + RoiLds<PPL>::kHJ; }
+  __device__ double* red() { return lds + RoiLds<PPL>::kRed; }
 
   // Shape table of state st in LDS: entry c < 32 = column c, 32 + r = row r.
   __device__ void table(const Model& m, const PeakState& st) {
@@ -564,10 +627,14 @@ __device__ inline double anscombe(double v) {
 // Returns 1 when MLE/FWLS meet fi <= 0: pixels before the first such pixel
 // (ROI order) are refreshed, the rest keep their cached fi, and the error is
 // not updated.  Adds n_neg_fi into neg_fi.
-template <int PPL>
+//
+// EM >= 0 fixes the error model at compile time (the kernels' MLE
+// instances); EM < 0 reads it from m.
+template <int PPL, int EM = -1>
 __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double& err, int& neg_fi) {
   const int l = lane_id();
-  const bool stop_on_neg = (m.errm == EM_MLE || m.errm == EM_FWLS);
+  const int errm = EM >= 0 ? EM : m.errm;
+  const bool stop_on_neg = (errm == EM_MLE || errm == EM_FWLS);
   double sum = 0.0;
   bool stopped = false;  // uniform: a slot held the ROI's first fi <= 0
 #pragma unroll
@@ -582,18 +649,18 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double
       if (o && l < lim) {
         double xk = c.x(s);
         double e;
-        if (m.errm == EM_MLE) {
+        if (errm == EM_MLE) {
           e = 2.0 * ((t - xk) - xk * log(t / xk));
-        } else if (m.errm == EM_ALS) {
+        } else if (errm == EM_ALS) {
           double a = anscombe(t);
           zero = (a == 0.0) && ((c.stale >> s) & 1u);
           t = a;
           double d = a - asx[c.k(s)];
           e = d * d;
-        } else if (m.errm == EM_LS) {
+        } else if (errm == EM_LS) {
           double d = t - xk;
           e = d * d;
-        } else if (m.errm == EM_DWLS) {
+        } else if (errm == EM_DWLS) {
           double d = t - xk;
           e = d * d / xk;
         } else {
@@ -604,7 +671,7 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double
         c.stale &= ~(1u << s);
         sum += e;
       }
-      if (m.errm == EM_ALS) neg_fi += __popcll(__ballot(zero));
+      if (errm == EM_ALS) neg_fi += __popcll(__ballot(zero));
       if (bad) stopped = true;
     }
   }
@@ -619,9 +686,10 @@ __device__ int eval_error(const Model& m, Roi<PPL>& c, const double* asx, double
 // daoCalcJH* (dao_fit.c:48-958): J and the upper triangle of H, packed row by
 // row, reduced over the wave into the slice's J/H area (J first).  Reads the
 // cached transformed fit tfi and the shape table, which must hold st's.
-template <int PPL, int SHAPE>
+template <int PPL, int SHAPE, int EM = -1>
 __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const double* asx) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
+  const int errm = EM >= 0 ? EM : m.errm;
   double a1 = st.prm[P_HEIGHT];
   double wx = st.prm[P_XW];
   double wy = (SHAPE == SH_3D || SHAPE == SH_Z) ? st.prm[P_YW] : wx;  // 2D/2DFixed: XWIDTH twice
@@ -673,21 +741,21 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
         g[4] = 1.0;
       }
       const double rf = 1.0 / fi;
-      double sc = (m.errm == EM_ALS) ? rq * 2.0 * rf : rq;
+      double sc = (errm == EM_ALS) ? rq * 2.0 * rf : rq;
 #pragma unroll
       for (int a = 0; a < N; a++) g[a] *= sc;
       double wj, wh;
-      if (m.errm == EM_MLE) {
+      if (errm == EM_MLE) {
         const double xr = xk * rf;
         wj = 2.0 * (1.0 - xr);
         wh = 2.0 * xr * rf;
-      } else if (m.errm == EM_ALS) {
+      } else if (errm == EM_ALS) {
         wj = fi - asx[c.k(s)];
         wh = 1.0;
-      } else if (m.errm == EM_LS) {
+      } else if (errm == EM_LS) {
         wj = fi - xk;
         wh = 1.0;
-      } else if (m.errm == EM_DWLS) {
+      } else if (errm == EM_DWLS) {
         const double rx = 1.0 / xk;
         wj = (fi - xk) * rx;
         wh = rx;
@@ -705,18 +773,12 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
       }
     }
   }
-  double* out = c.hj();
+  double v[N + N * (N + 1) / 2];  // J, then H packed row by row
 #pragma unroll
-  for (int a = 0; a < N; a++) {
-    const double v = wsum(j_acc[a]);
-    if (lane_id() == 0) out[a] = v;
-  }
+  for (int a = 0; a < N; a++) v[a] = j_acc[a];
 #pragma unroll
-  for (int a = 0; a < N * (N + 1) / 2; a++) {
-    const double v = wsum(h_acc[a]);
-    if (lane_id() == 0) out[N + a] = v;
-  }
-  wave_sync_lds();
+  for (int a = 0; a < N * (N + 1) / 2; a++) v[N + a] = h_acc[a];
+  wsum_many(v, c.red(), c.hj());
 }
 
 // ---------------------------------------------------------------------------
@@ -734,7 +796,7 @@ __device__ void calc_jh(const Model& m, Roi<PPL>& c, const PeakState& st, const 
 // 32-bit DPP move (as dpp_mov).
 template <int CTRL>
 __device__ inline float dpp_movf(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
 __device__ inline float lane_valuef(float v, int lane) {
@@ -749,6 +811,47 @@ __device__ inline double wsumf(float v) {
   v += dpp_movf<0x140>(v);
   return ((double)lane_valuef(v, 0) + (double)lane_valuef(v, 16)) +
          ((double)lane_valuef(v, 32) + (double)lane_valuef(v, 48));
+}
+
+template <int CTRL, int K, int H>
+__device__ inline void fold_levelf(const float (&v)[K], float (&out)[H], bool upper) {
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    const float lo = pick(v, i), hi = pick(v, i + H);
+    const float keep = upper ? hi : lo, send = upper ? lo : hi;
+    out[i] = keep + dpp_movf<CTRL>(send);
+  }
+}
+
+// wsumf of each of K values per lane into out[0, K) (LDS), as wsum_many:
+// FP32 sums inside each 16-lane row, the four rows combined in FP64, each
+// bit for bit wsumf's.
+template <int K>
+__device__ void wsumf_many(const float (&v)[K], double* red, double* out) {
+  using F = Fold<K>;
+  static_assert(F::H2 <= kRedSlots, "reduction slots");
+  float* const rf = reinterpret_cast<float*>(red);
+  const int l = lane_id();
+  float a[F::H1];
+  fold_levelf<0xB1>(v, a, (l & 1) != 0);
+  float b[F::H2];
+  fold_levelf<0x4E>(a, b, (l & 2) != 0);
+#pragma unroll
+  for (int j = 0; j < F::H2; j++) rf[j * kWave + l] = b[j];
+  wave_sync_lds();
+  if (l < K) {
+    const int i = l % F::H1;
+    const int lane0 = ((i / F::H2) << 1) | (l / F::H1);
+    const float* q = rf + (i % F::H2) * kWave + lane0;
+    double r[4];
+#pragma unroll
+    for (int row = 0; row < 4; row++) {
+      const float* p = q + row * 16;
+      r[row] = (double)((p[0] + p[4]) + (p[8] + p[12]));
+    }
+    out[l] = (r[0] + r[1]) + (r[2] + r[3]);
+  }
+  wave_sync_lds();
 }
 
 // h(u) = u - ln(1 + u) >= 0 for u > -1; the MLE deviance term is
@@ -783,7 +886,8 @@ struct RoiLdsM {
   static constexpr int kTab = kScm + PPL * kWave / 2;
   static constexpr int kHJ = kTab + 2 * kWave;
   static constexpr int kD0 = kHJ + 32;
-  static constexpr int kDoubles = kD0 + PPL * kWave;
+  static constexpr int kRed = kD0 + PPL * kWave;  // wsum_many's partial sums
+  static constexpr int kDoubles = kRed + kRedSlots * kWave;
 };
 
 template <int PPL>
@@ -813,6 +917,7 @@ struct RoiM {
   __device__ float& scm(int s) { return reinterpret_cast<float*>(lds + RoiLdsM<PPL>::kScm)[s * kWave + lane_id()]; }
   __device__ double2* tab() { return reinterpret_cast<double2*>(lds + RoiLdsM<PPL>::kTab); }
   __device__ double* hj() { return lds + RoiLdsM<PPL>::kHJ; }
+  __device__ double* red() { return lds + RoiLdsM<PPL>::kRed; }
   __device__ double& d0(int s) { return lds[RoiLdsM<PPL>::kD0 + s * kWave + lane_id()]; }
 
   __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
@@ -925,7 +1030,7 @@ struct RoiM {
 
 // MLE deviance over the held ROI (mFitCalcErr, multi_fit.c:111-195), mixed
 // precision; same early stop on fi <= 0 as eval_error.
-template <int PPL>
+template <int PPL, int EM = -1>
 __device__ int eval_error(const Model& m, RoiM<PPL>& c, const double* asx, double& err, int& neg_fi) {
   const int l = lane_id();
   double sum = 0.0;
@@ -1030,7 +1135,7 @@ __device__ int eval_delta(const Model& m, RoiM<PPL>& c, double& delta, int& neg_
 
 // daoCalcJH* for MLE (dao_fit.c:48-958) in FP32; results to the slice's J/H
 // area in FP64.  The shape table must hold st's.
-template <int PPL, int SHAPE>
+template <int PPL, int SHAPE, int EM = -1>
 __device__ void calc_jh(const Model& m, RoiM<PPL>& c, const PeakState& st, const double*) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
   const float a1 = (float)st.prm[P_HEIGHT];
@@ -1101,18 +1206,8 @@ __device__ void calc_jh(const Model& m, RoiM<PPL>& c, const PeakState& st, const
       }
     }
   }
-  double* out = c.hj();
-#pragma unroll
-  for (int a = 0; a < N; a++) {
-    const double v = wsum(j_acc[a]);
-    if (lane_id() == 0) out[a] = v;
-  }
-#pragma unroll
-  for (int a = 0; a < N * (N + 1) / 2; a++) {
-    const double v = wsumf(h_acc[a]);
-    if (lane_id() == 0) out[N + a] = v;
-  }
-  wave_sync_lds();
+  wsum_many(j_acc, c.red(), c.hj());
+  wsumf_many(h_acc, c.red(), c.hj() + N);
 }
 
 template <int N>
@@ -1270,7 +1365,7 @@ __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
 // loads into jr (nsnap snapshots, see kJournalSnaps; the first one into
 // `stage` instead when given), so that it can be undone exactly; a turn that
 // cannot be undone or leave its box (its box is the whole frame) needs none.
-template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1>
+template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1, int EM = -1>
 __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
                        double* slice, int p, int4 box, int* d, double* jr, int& nsnap, double* stage = nullptr) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
@@ -1303,12 +1398,12 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   constexpr bool kDelta = std::is_same<R, RoiM<PPL>>::value;
   double wJ[N];
   double e;
-  if (eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]) == 0) {
+  if (eval_error<PPL, EM>(m, c, px.asx, e, d[C_NEG_FI]) == 0) {
     w.err = e;
     if constexpr (kDelta) keep_start<PPL>(c);
   }
   const double start = w.err;
-  calc_jh<PPL, SHAPE>(m, c, w, px.asx);
+  calc_jh<PPL, SHAPE, EM>(m, c, w, px.asx);
   const double* J = c.hj();
   const double* H = c.hj() + N;
   c.stamp(w, -1);
@@ -1370,11 +1465,11 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
         bad_e = eval_delta<PPL>(m, c, dl, d[C_NEG_FI]);
         e = start + dl;
       } else {
-        bad_e = eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]);
+        bad_e = eval_error<PPL, EM>(m, c, px.asx, e, d[C_NEG_FI]);
         dl = e - start;
       }
     } else {
-      bad_e = eval_error<PPL>(m, c, px.asx, e, d[C_NEG_FI]);
+      bad_e = eval_error<PPL, EM>(m, c, px.asx, e, d[C_NEG_FI]);
       dl = e - start;
     }
     if (bad_e != 0) {
@@ -1788,7 +1883,7 @@ __device__ inline void fpw_set_cur(int* cur, int v) {
 // and never wait on later ones): the finished ops above it are undone newest
 // first, the turn gets the whole frame as its box (it then runs alone and
 // cannot escape again), and the run resumes from it.
-template <int PPL, int SHAPE, int NW, bool MP>
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
 __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int n, const int* list, FpwShared<NW>& sh,
                         double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
@@ -1828,7 +1923,7 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int 
       double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
       double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
       if (kind == OP_LM) {
-        r = lm_turn<PPL, SHAPE, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
+        r = lm_turn<PPL, SHAPE, R, MemL1, EM>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
       } else {
         recenter_turn<PPL, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       }
@@ -1891,7 +1986,7 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int 
   }
 }
 
-template <int PPL, int SHAPE, int NW, bool MP>
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
 __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
   __shared__ FpwShared<NW> sh;
@@ -1928,7 +2023,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
       a.pk.bk[p] = r;
     }
     __syncthreads();
-    fpw_run<PPL, SHAPE, NW, MP>(a, f, fr, OP_LM, n, a.run_list, sh, slice);
+    fpw_run<PPL, SHAPE, NW, MP, EM>(a, f, fr, OP_LM, n, a.run_list, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
     const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum, a.run_list, n, a.op_peak);
     if (nr > 0) {
@@ -1945,7 +2040,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
         a.box[slot] = b;
       }
       __syncthreads();
-      fpw_run<PPL, SHAPE, NW, MP>(a, f, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
+      fpw_run<PPL, SHAPE, NW, MP, EM>(a, f, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
     }
   }
   __syncthreads();
@@ -2105,7 +2200,7 @@ __device__ bool team_wait(const FitArgs& a, int f, TeamCtl* ctl, const Frame& fr
 
 // One wave takes and runs ops of generation g until none is left, a
 // transition is claimed or an op escapes.  Its seat reads kActive meanwhile.
-template <int PPL, int SHAPE, bool MP>
+template <int PPL, int SHAPE, bool MP, int EM = -1>
 __device__ void team_ops(const FitArgs& a, int f, const Frame& fr, TeamCtl* ctl, int g, int kind, int n, int seat,
                          TeamShared& sh, double* slice) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
@@ -2132,7 +2227,7 @@ __device__ void team_ops(const FitArgs& a, int f, const Frame& fr, TeamCtl* ctl,
     double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
     double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
     if (kind == OP_LM) {
-      r = lm_turn<PPL, SHAPE, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
+      r = lm_turn<PPL, SHAPE, R, MemDev, EM>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
     } else {
       recenter_turn<PPL, R, MemDev>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
     }
@@ -2295,7 +2390,7 @@ __device__ DAO_TEAM_COLD void team_transition(const FitArgs& a, const Frame& fr,
 
 // Workgroup b of the launch belongs to frame team_block_frame[b]; its waves
 // use seats team_block_seat[b] + wave.  Dynamic LDS as k_fit_frames.
-template <int PPL, int SHAPE, int NW, bool MP>
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
 __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
   static_assert(NW <= kTeamSeats, "seats per workgroup");
   constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
@@ -2337,7 +2432,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
     }
     g = gen;
     if (kind == kTeamDone) break;
-    team_ops<PPL, SHAPE, MP>(a, f, fr, ctl, g, kind, n, seat, sh, slice);
+    team_ops<PPL, SHAPE, MP, EM>(a, f, fr, ctl, g, kind, n, seat, sh, slice);
     __syncthreads();
   }
   __syncthreads();
@@ -2373,38 +2468,42 @@ static inline dim3 flat_grid(long long n) { return dim3((unsigned)((n + 255) / 2
 
 // Launch k_fit_frames<PPL, SHP, NW, MP> for nframes frames with a.m.waves
 // waves per workgroup (0: NW; at most NW) and their LDS slices.
-template <int PPL, int SHP, int NW, bool MP>
+template <int PPL, int SHP, int NW, bool MP, int EM = -1>
 void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
   const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
   const size_t bytes = sizeof(double) * (size_t)nw * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;  // larger dynamic LDS must be enabled per kernel
   if (bytes > allowed) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP, EM>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
       return;  // the launch below would fail; the caller's error check reports it
     allowed = bytes;
   }
-  k_fit_frames<PPL, SHP, NW, MP><<<nframes, nw * 64, bytes, st>>>(a);
+  k_fit_frames<PPL, SHP, NW, MP, EM><<<nframes, nw * 64, bytes, st>>>(a);
 }
 
 // Launch k_fit_team<PPL, SHP, NW, MP> over nblocks workgroups (the team
 // tables of `a` map them to frames and seats), waves as k_fit_frames.
-template <int PPL, int SHP, int NW, bool MP>
+template <int PPL, int SHP, int NW, bool MP, int EM = -1>
 void launch_fit_team_t(const FitArgs& a, int nblocks, hipStream_t st) {
   const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
   const size_t bytes = sizeof(double) * (size_t)nw * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;
   if (bytes > allowed) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_team<PPL, SHP, NW, MP>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_team<PPL, SHP, NW, MP, EM>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
       return;
     allowed = bytes;
   }
-  k_fit_team<PPL, SHP, NW, MP><<<nblocks, nw * 64, bytes, st>>>(a);
+  k_fit_team<PPL, SHP, NW, MP, EM><<<nblocks, nw * 64, bytes, st>>>(a);
 }
 
-// k_fit_frames / k_fit_team launchers (dao_fit_fp64.hip, dao_fit_mixed.hip)
+// k_fit_frames / k_fit_team launchers (dao_fit_fp64.hip, dao_fit_mixed.hip,
+// dao_team_*.hip); FP64 with the MLE error model has its own instances
+// (dao_fit_fp64_mle.hip, dao_team_fp64_mle.hip)
 void launch_fit_frames_fp64(const FitArgs& a, int nframes, hipStream_t st);
+void launch_fit_frames_fp64_mle(const FitArgs& a, int nframes, hipStream_t st);
+void launch_fit_team_fp64_mle(const FitArgs& a, int nblocks, hipStream_t st);
 void launch_fit_frames_mixed(const FitArgs& a, int nframes, hipStream_t st);
 void launch_fit_team_fp64(const FitArgs& a, int nblocks, hipStream_t st);
 void launch_fit_team_mixed(const FitArgs& a, int nblocks, hipStream_t st);
