@@ -95,6 +95,10 @@ def parse():
                    help="workers of the serving leg's latency set-up")
     p.add_argument("--serve-slots", type=int, default=32,
                    help="frame slots per worker in the serving leg's latency set-up")
+    p.add_argument("--serve-setups", default="resident,throughput,latency",
+                   help="serving set-ups to run: resident (frames admitted into a running kernel), "
+                        "throughput, latency (a launch per batch)")
+    p.add_argument("--resident-slots", type=int, default=64, help="frame slots of the resident server")
     p.add_argument("--serve-team", type=int, default=250,
                    help="peaks per workgroup in the serving leg's latency set-up")
     p.add_argument("--no-dense", action="store_true",
@@ -762,26 +766,36 @@ def serve_leg(args, precision, log):
     from storm_analysis_amd.sa_library import dao_serve
     from storm_analysis_amd.diagnostics import serve_sim
     pools = serve_sim.make_pools((256, 512, 1024))
-    setups = {"throughput": dict(workers=8, slots=32, team=0, margin=2),
+    setups = {"resident": dict(slots=args.resident_slots, margin=2),
+              "throughput": dict(workers=8, slots=32, team=0, margin=2),
               "latency": dict(workers=args.serve_workers, slots=args.serve_slots, team=args.serve_team, margin=1)}
+    setups = {k: v for k, v in setups.items() if k in args.serve_setups.split(",")}
     base = dict(max_shape=(1024, 1024), max_peaks=4000, model="Z", roi_size=ROI, precision=precision, **zkw())
     out = {"server": {"max_shape": [1024, 1024], "max_peaks": 4000, "precision": precision, "setups": setups},
            "cameras": 16, "rates_fps": [50, 400], "sizes": [256, 512, 1024], "runs": []}
+
+    def server(name, kw):
+        # resident: frames admitted into one running kernel (ResidentServer);
+        # the others: a launch per batch of queued frames (FitServer)
+        return dao_serve.ResidentServer(**kw) if name == "resident" else dao_serve.FitServer(**kw)
+
     for name, su in setups.items():
         srv_kw = dict(base, **su)
-        srv = dao_serve.FitServer(**srv_kw)  # warm the kernels once
+        srv = server(name, srv_kw)  # warm the kernels once
         serve_sim.run(srv, duration_s=0.2, load=0.05, pools=pools)
         srv.close()
         for load in [float(x) for x in args.serve_loads.split(",") if x]:
-            srv = dao_serve.FitServer(**srv_kw)
+            srv = server(name, srv_kw)
             try:
                 summ = serve_sim.run(srv, duration_s=args.serve_seconds, load=load, pools=pools)
             finally:
                 srv.close()
             summ["setup"] = name
-            log("serving %s load %.2f: %d frames, p50 %.1f ms, p99 %.1f ms, %.0f fits/s (offered %.0f emitters/s)"
+            log("serving %s load %.2f: %d frames, p50 %.1f ms, p99 %.1f ms, %.0f fits/s (offered %.0f emitters/s)%s"
                 % (name, load, summ["requests"], summ["p50_ms"], summ["p99_ms"], summ["fits_per_s"],
-                   summ["offered_emitters_per_s"]))
+                   summ["offered_emitters_per_s"],
+                   "".join("; %s p50 %.1f p99 %.1f ms" % (k, v["p50_ms"], v["p99_ms"])
+                           for k, v in summ.get("by_size", {}).items())))
             out["runs"].append(summ)
     return out
 

@@ -6,17 +6,9 @@
 #include <memory>
 #include <vector>
 
-#include "dao_engine.h"
-#include "../../include/dao_batch_amd.h"
+#include "dao_batch_handle.h"
 
 using dao::Engine;
-
-struct daoBatch {
-  std::unique_ptr<Engine> eng;
-  std::vector<int> counters;  // [F][8], host totals not yet handed out
-  double* d_tmp = nullptr;
-  size_t tmp_bytes = 0;
-};
 
 namespace {
 
@@ -41,7 +33,9 @@ void check_frame_index(const Engine& e, int f) {
   if (f < 0 || f >= e.F) dao::fail(dao::kErrArg, "frame index out of range");
 }
 
-int peak_format(const char* p_type) {
+}  // namespace
+
+int dao_peak_format(const char* p_type) {
   if (p_type != nullptr) {
     if (!std::strcmp(p_type, "finder")) return dao::PF_FINDER;
     if (!std::strcmp(p_type, "testing")) return dao::PF_TESTING;
@@ -50,6 +44,10 @@ int peak_format(const char* p_type) {
   std::printf("Unrecognized peak type '%s'!\n", p_type ? p_type : "(null)");
   return -1;
 }
+
+namespace {
+
+int peak_format(const char* p_type) { return dao_peak_format(p_type); }
 
 }  // namespace
 

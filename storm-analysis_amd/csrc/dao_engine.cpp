@@ -490,6 +490,33 @@ int Engine::team_blocks(std::vector<int>& tab, int& nseats) const {
   return (int)bf.size();
 }
 
+FitArgs Engine::fitter_args(int max_more) {
+  if (d_sweeps == nullptr) {
+    d_sweeps = dalloc<int>(F);
+    d_flist = dalloc<int>(F);
+    d_err = dalloc<int>(F);
+  }
+  FitArgs a{};
+  a.m = m;
+  a.px = px;
+  a.pk = pk;
+  a.frames = d_fr;
+  a.roi_tab = d_roi;
+  a.op_peak = sc.op_peak;
+  a.run_list = d_run;
+  a.journal = d_journal;
+  a.box = sc.box;
+  a.op_delta = sc.op_delta;
+  a.op_done = sc.op_flags;
+  a.frame_counters = d_counters;
+  a.frame_sweeps = d_sweeps;
+  a.frame_list = nullptr;
+  a.max_more = max_more;
+  a.frame_err = d_err;
+  a.spin_limit = spin_limit > 0 ? spin_limit : kSpinLimit;
+  return a;
+}
+
 int Engine::fit_frames(int max_more) {
   std::vector<int> list;
   for (int f = 0; f < F; f++)

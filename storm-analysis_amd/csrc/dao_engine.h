@@ -203,6 +203,9 @@ class Engine {
   // A frame whose scheduler wait gave up is left empty (nfit = 0) and
   // flagged in `failed`; the other frames' results stand.
   int fit_frames(int max_more);
+  // Arguments of the frame fitters over this engine's state, without a
+  // frame list or team tables (the resident fitter takes frames itself).
+  FitArgs fitter_args(int max_more);
   std::vector<int> failed;   // per frame: failed in the last fit
   int failed_count = 0;
   long long spin_limit = 0;  // polls before a scheduler wait gives up (0: kSpinLimit)

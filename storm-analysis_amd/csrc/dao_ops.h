@@ -1883,9 +1883,13 @@ __device__ inline void fpw_set_cur(int* cur, int v) {
 // and never wait on later ones): the finished ops above it are undone newest
 // first, the turn gets the whole frame as its box (it then runs alone and
 // cannot escape again), and the run resumes from it.
-template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
-__device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int n, const int* list, FpwShared<NW>& sh,
-                        double* slice) {
+//
+// OP_NEWPEAK ops (daoNewPeaks) take op k's start row from rows (fmt's
+// layout, row k), OP_PEAKERROR ops compute a new peak's first error; neither
+// moves out of its box.
+template <int PPL, int SHAPE, int NW, bool MP, int EM, int kind>
+__device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int n, const int* list, FpwShared<NW>& sh,
+                        double* slice, const double* rows = nullptr, int fmt = 0) {
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < n; k += blockDim.x) a.op_done[fr.peak0 + k] = 0;
@@ -1922,10 +1926,15 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int 
       int r = 1, nsnap = 0;
       double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
       double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
-      if (kind == OP_LM) {
+      if constexpr (kind == OP_LM) {
         r = lm_turn<PPL, SHAPE, R, MemL1, EM>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
-      } else {
+      } else if constexpr (kind == OP_RECENTER) {
         recenter_turn<PPL, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
+      } else if constexpr (kind == OP_NEWPEAK) {
+        newpeak_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, rows + (long long)j * (fmt == PF_TEXT ? 7 : 4),
+                          fmt, j, d);
+      } else {
+        peakerror_turn<PPL>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       }
       if (r == 2) {
         if (lane == 0) atomicMin(&sh.esc, j);
@@ -1986,18 +1995,15 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int kind, int 
   }
 }
 
+// MultiFitter.doFit (dao_fit_c.py:332-356) of frame f by the calling
+// workgroup: sweeps (mFitIterateLM, multi_fit.c:1072-1349) and recentering
+// passes until no peak is RUNNING or a.max_more sweeps followed the first
+// (a.max_more = 0: exactly one sweep).  Counter increments accumulate in
+// sh.cnt; sh.failed is set if a scheduler wait gave up.  Returns the sweeps
+// run.
 template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
-__global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
-  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
-  __shared__ FpwShared<NW> sh;
-  extern __shared__ double lds_dyn[];
-  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
-  const int f = a.frame_list[blockIdx.x];
-  const Frame fr = a.frames[f];
+__device__ int fit_frame(const FitArgs& a, int f, const Frame& fr, FpwShared<NW>& sh, double* slice) {
   const Model& m = a.m;
-  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
-  if (threadIdx.x == 0) sh.failed = 0;
-  __syncthreads();
   int sweeps = 0;
   int n = 0;  // RUNNING peaks at the sweep's start: a.run_list[peak0 .. peak0 + n)
   for (int it = 0;; it++) {
@@ -2023,7 +2029,7 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
       a.pk.bk[p] = r;
     }
     __syncthreads();
-    fpw_run<PPL, SHAPE, NW, MP, EM>(a, f, fr, OP_LM, n, a.run_list, sh, slice);
+    fpw_run<PPL, SHAPE, NW, MP, EM, OP_LM>(a, f, fr, n, a.run_list, sh, slice);
     // mFitRecenterPeaks (multi_fit.c:1644-1690)
     const int nr = fpw_list(a, fr, SEL_RECENTER, sh.wsum, a.run_list, n, a.op_peak);
     if (nr > 0) {
@@ -2040,10 +2046,25 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
         a.box[slot] = b;
       }
       __syncthreads();
-      fpw_run<PPL, SHAPE, NW, MP, EM>(a, f, fr, OP_RECENTER, nr, a.op_peak, sh, slice);
+      fpw_run<PPL, SHAPE, NW, MP, EM, OP_RECENTER>(a, f, fr, nr, a.op_peak, sh, slice);
     }
   }
   __syncthreads();
+  return sweeps;
+}
+
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
+__global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
+  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
+  __shared__ FpwShared<NW> sh;
+  extern __shared__ double lds_dyn[];
+  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
+  const int f = a.frame_list[blockIdx.x];
+  const Frame fr = a.frames[f];
+  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.failed = 0;
+  __syncthreads();
+  const int sweeps = fit_frame<PPL, SHAPE, NW, MP, EM>(a, f, fr, sh, slice);
   if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
     atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
   if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;

@@ -27,14 +27,17 @@ Frames share no state in the fitter, so a frame's result does not depend on
 which other frames share its launch (to the parity bar; bit-identical unless
 a reserved box overflowed and a rollback ran).
 """
+import collections
+import ctypes
 import queue
 import threading
 import time
 
 import numpy
+from numpy.ctypeslib import ndpointer
 
 from . import dao_batch
-from .dao_fit_c import MultiFitterException
+from .dao_fit_c import MultiFitterException, formatPeaksGaussianPSF, lastLibraryError, raiseOnLibraryError
 
 # device bytes per slot pixel (two 32-byte pixel records + the Anscombe
 # image) and per slot peak (two 96-byte peak records + scheduling scratch),
@@ -96,6 +99,15 @@ def _invalid_input(image, background, rows):
     if rows is not None and rows.shape[0] and not _all_finite(rows):
         return "a start peak holds non-finite values"
     return None
+
+
+def latency_by_size(done):
+    """p50 / p99 latency (ms) of the completed requests per frame size."""
+    by_size = {}
+    for r in done:
+        by_size.setdefault("%dx%d" % (r.image.shape[0], r.image.shape[1]), []).append(r.latency() * 1e3)
+    return {k: {"frames": len(v), "p50_ms": float(numpy.percentile(v, 50)), "p99_ms": float(numpy.percentile(v, 99))}
+            for k, v in sorted(by_size.items())}
 
 
 class FitRequest(object):
@@ -267,7 +279,241 @@ class FitServer(object):
         conv = sum(int((r.result["status"] == 1).sum()) for r in done if r.result is not None)
         return {"requests": len(done), "p50_ms": float(numpy.percentile(lat, 50)),
                 "p99_ms": float(numpy.percentile(lat, 99)), "max_ms": float(lat.max()),
+                "by_size": latency_by_size(done),
                 "converged": conv, "fits_per_s": conv / (t1 - t0), "frames_per_s": len(done) / (t1 - t0),
                 "launches": self.launches, "mean_batch": float(numpy.mean([r.batch_size for r in done])),
                 "workers": self.workers, "slots": self.slots, "slot_bytes": self.slot_bytes,
+                "errors": sum(1 for r in done if r.error is not None)}
+
+
+# ---------------------------------------------------------------------------
+# the resident fitter: frames admitted into a running kernel
+# ---------------------------------------------------------------------------
+
+_res_lib = None
+
+
+def loadResidentC():
+    """libdao_fit.so with the daoResident* entry points typed (include/dao_resident_amd.h)."""
+    global _res_lib
+    if _res_lib is None:
+        lib = dao_batch.loadBatchC()
+        vp = ctypes.c_void_p
+        i32p = ndpointer(dtype=numpy.int32)
+        f64p = ndpointer(dtype=numpy.float64)
+        lib.daoResidentCreate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.daoResidentCreate.restype = vp
+        lib.daoResidentDestroy.argtypes = [vp]
+        lib.daoResidentSubmit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
+                                          ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        lib.daoResidentSubmit.restype = ctypes.c_int
+        lib.daoResidentWait.argtypes = [vp, ctypes.c_double, ctypes.c_int, i32p]
+        lib.daoResidentWait.restype = ctypes.c_int
+        lib.daoResidentResult.argtypes = [vp, ctypes.c_int, f64p, i32p, i32p, f64p]
+        lib.daoResidentResult.restype = ctypes.c_int
+        lib.daoResidentStats.argtypes = [vp, ndpointer(dtype=numpy.int64)]
+        _res_lib = raiseOnLibraryError(lib, ("daoResidentSubmit", "daoResidentWait"))
+    return _res_lib
+
+
+class ResidentServer(object):
+    """
+    Continuous batching inside one running kernel.  `slots` frame slots of
+    max_shape (height, width) and max_peaks peaks (fewer if hbm_budget_gb
+    holds fewer) on one device; a resident kernel (include/dao_resident_amd.h)
+    whose workgroups take each request as soon as one is free, fit it
+    (new image, background, start peaks, doFit) and write its peaks to
+    pinned host memory.  A request waits in a host queue only while every
+    slot is taken; submitting is a few host memory writes, with no launch.
+    Frames share no state: each result equals the frame fitted alone
+    (BatchFitter) bit for bit.  MLE only; the remaining keywords are
+    BatchFitter's (model, precision, roi_size, wx_params, ...).
+    """
+    def __init__(self, slots=64, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None, max_iterations=2000,
+                 margin=-1, workgroups=0, idle_seconds=0.5, wait_limit=0, **fitter_kw):
+        self.clib = loadResidentC()
+        self.max_shape = tuple(max_shape)
+        self.max_peaks = max_peaks
+        roi_size = fitter_kw.get("roi_size", 18)
+        _, slots = pool_size(1, slots, max_shape, max_peaks, hbm_budget_gb, roi_size)
+        self.slots = slots
+        self.workers = 1
+        self.slot_bytes = slot_bytes(max_shape, max_peaks, roi_size)
+        self.bf = dao_batch.BatchFitter([self.max_shape] * slots, [max_peaks] * slots, **fitter_kw)
+        self.bf.setTuning(0, margin, 0)
+        self.bf.setWaitLimit(wait_limit)
+        self.handle = self.clib.daoResidentCreate(self.bf.handle, workgroups, max_iterations, idle_seconds)
+        if not self.handle:
+            code, msg = lastLibraryError(self.clib)
+            self.bf.cleanup()
+            raise MultiFitterException("daoResidentCreate failed (code %d): %s" % (code, msg))
+        self._lock = threading.Lock()
+        self.free = collections.deque(range(slots))
+        self.pending = collections.deque()
+        self.inflight = {}
+        self.completed = []
+        self.launches = 0
+        self._stop = False
+        self._poller = threading.Thread(target=self._poll, daemon=True)
+        self._poller.start()
+
+    def submit(self, image, background, rows, peaks_type="hdf5", camera=0, t_arrival=None):
+        """Queue one frame; returns its FitRequest (results as BatchFitter.getPeaks)."""
+        h, w = image.shape[0], image.shape[1]
+        if h > self.max_shape[0] or w > self.max_shape[1]:
+            raise MultiFitterException("frame %dx%d exceeds the slot size %s" % (h, w, self.max_shape))
+        if isinstance(rows, dict):
+            rows = formatPeaksGaussianPSF(rows, peaks_type)
+        if rows is not None and rows.shape[0] > self.max_peaks:
+            raise MultiFitterException("%d peaks exceed the slot's %d" % (rows.shape[0], self.max_peaks))
+        r = FitRequest(image, background, rows, peaks_type, camera,
+                       time.perf_counter() if t_arrival is None else t_arrival)
+        # device arrays are checked by the kernel as it loads them (a check
+        # here would be a kernel of its own, queued behind the resident one)
+        bad = None if hasattr(image, "is_cuda") and image.is_cuda else _invalid_input(image, background, rows)
+        if bad is not None:
+            r.error = MultiFitterException("request refused: " + bad)
+            r.t_start = r.t_done = time.perf_counter()
+            r.batch_size = 0
+            with self._lock:
+                self.completed.append(r)
+            r.done.set()
+            return r
+        with self._lock:
+            if self.free:
+                slot = self.free.popleft()
+                if not self._start(slot, r):
+                    self.free.append(slot)
+            else:
+                self.pending.append(r)
+        return r
+
+    def _next(self, slot):
+        # with self._lock held: the slot's next request, or back to the pool
+        while self.pending:
+            if self._start(slot, self.pending.popleft()):
+                return
+        self.free.append(slot)
+
+    def setWaitLimit(self, polls):
+        """Polls after which a scheduler wait gives up and fails its frame's request (0: default)."""
+        self.bf.setWaitLimit(polls)
+
+    def _start(self, slot, r):
+        # with self._lock held; False (request failed, slot unused) if refused
+        img, dev_i, ki = dao_batch._ptr(r.image, numpy.float64)
+        bg, dev_b, kb = dao_batch._ptr(r.background, numpy.float64)
+        n = 0 if r.rows is None else int(r.rows.shape[0])
+        if n > 0:
+            rw, dev_r, kr = dao_batch._ptr(r.rows, numpy.float64)
+        else:
+            rw, dev_r, kr = None, dev_i, None
+        if not (dev_i == dev_b == dev_r):
+            r.error = MultiFitterException("image, background and peaks must all be host or all device arrays")
+            r.t_start = r.t_done = time.perf_counter()
+            self.completed.append(r)
+            r.done.set()
+            return False
+        r._keep = (ki, kb, kr)
+        ticket = ctypes.c_int64(0)
+        r.t_start = time.perf_counter()
+        r.worker = 0
+        r.batch_size = 1
+        try:
+            self.clib.daoResidentSubmit(self.handle, slot, r.image.shape[1], r.image.shape[0], img, bg, rw, n,
+                                        r.peaks_type.encode(), dev_i, ctypes.byref(ticket))
+        except MultiFitterException as e:  # this request only; the slot stays free
+            r.error = e
+            r.t_done = time.perf_counter()
+            r._keep = None
+            self.completed.append(r)
+            r.done.set()
+            return False
+        self.inflight[slot] = r
+        return True
+
+    def _finish(self, slot):
+        # with self._lock held
+        r = self.inflight.pop(slot)
+        cap = self.max_peaks
+        out = numpy.zeros((max(cap, 1), 9))
+        status = numpy.zeros(max(cap, 1), dtype=numpy.int32)
+        counters = numpy.zeros(8, dtype=numpy.int32)
+        info = numpy.zeros(4)
+        n = self.clib.daoResidentResult(self.handle, slot, out, status, counters, info)
+        if n < 0:
+            code, msg = lastLibraryError(self.clib)
+            r.error = dao_batch.FrameFitError("resident fit failed (code %d): %s" % (n, msg), [slot])
+        else:
+            res = {c: out[:n, i].copy() for i, c in enumerate(dao_batch.PEAK_COLUMNS)}
+            res["iterations"] = res["iterations"].astype(numpy.int32)
+            res["status"] = status[:n].copy()
+            r.result = res
+        r.counters = dict(zip(dao_batch.COUNTER_NAMES, counters.tolist()))
+        r.sweeps = int(info[0])
+        r.device_seconds = float(info[1])
+        r.load_seconds = float(info[2])
+        r.t_done = time.perf_counter()
+        r._keep = None
+        self.completed.append(r)
+        r.done.set()
+        self._next(slot)
+
+    def _poll(self):
+        slots = numpy.zeros(self.slots, dtype=numpy.int32)
+        while True:
+            with self._lock:
+                if self._stop and not self.inflight and not self.pending:
+                    return
+            try:
+                k = self.clib.daoResidentWait(self.handle, 0.05, self.slots, slots)
+            except MultiFitterException as e:
+                with self._lock:
+                    for r in list(self.inflight.values()) + list(self.pending):
+                        r.error = e
+                        r.t_done = time.perf_counter()
+                        self.completed.append(r)
+                        r.done.set()
+                    self.inflight.clear()
+                    self.pending.clear()
+                return
+            if k > 0:
+                with self._lock:
+                    for slot in slots[:k].tolist():
+                        self._finish(slot)
+
+    def stats(self):
+        out = numpy.zeros(2, dtype=numpy.int64)
+        self.clib.daoResidentStats(self.handle, out)
+        return {"requests": int(out[0]), "kernel_launches": int(out[1])}
+
+    def close(self):
+        with self._lock:
+            self._stop = True
+        self._poller.join()
+        if self.handle:
+            self.launches = self.stats()["kernel_launches"]
+            self.clib.daoResidentDestroy(self.handle)
+            self.handle = None
+        self.bf.cleanup()
+
+    def summary(self, t_begin=None):
+        """Latency percentiles (ms), overall and per frame size, and throughput of the completed requests."""
+        with self._lock:
+            done = list(self.completed)
+        if not done:
+            return {"requests": 0}
+        lat = numpy.array([r.latency() for r in done]) * 1e3
+        t0 = min(r.t_arrival for r in done) if t_begin is None else t_begin
+        t1 = max(r.t_done for r in done)
+        conv = sum(int((r.result["status"] == 1).sum()) for r in done if r.result is not None)
+        dev = [r.device_seconds * 1e3 for r in done if getattr(r, "device_seconds", None) is not None]
+        return {"requests": len(done), "p50_ms": float(numpy.percentile(lat, 50)),
+                "p99_ms": float(numpy.percentile(lat, 99)), "max_ms": float(lat.max()),
+                "by_size": latency_by_size(done),
+                "device_fit_ms_p50": float(numpy.percentile(dev, 50)) if dev else None,
+                "device_fit_ms_p99": float(numpy.percentile(dev, 99)) if dev else None,
+                "converged": conv, "fits_per_s": conv / (t1 - t0), "frames_per_s": len(done) / (t1 - t0),
+                "kernel_launches": self.stats()["kernel_launches"] if self.handle else self.launches,
+                "slots": self.slots, "slot_bytes": self.slot_bytes,
                 "errors": sum(1 for r in done if r.error is not None)}
