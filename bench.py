@@ -99,6 +99,9 @@ def parse():
                    help="serving set-ups to run: resident (frames admitted into a running kernel), "
                         "throughput, latency (a launch per batch)")
     p.add_argument("--resident-slots", type=int, default=64, help="frame slots of the resident server")
+    p.add_argument("--resident-team", type=int, default=64,
+                   help="peaks per workgroup of the resident server's team set-up (a 512^2 frame of 1000 "
+                        "emitters gets 16 workgroups)")
     p.add_argument("--serve-team", type=int, default=250,
                    help="peaks per workgroup in the serving leg's latency set-up")
     p.add_argument("--no-dense", action="store_true",
@@ -767,6 +770,7 @@ def serve_leg(args, precision, log):
     from storm_analysis_amd.diagnostics import serve_sim
     pools = serve_sim.make_pools((256, 512, 1024))
     setups = {"resident": dict(slots=args.resident_slots, margin=2),
+              "resident_team": dict(slots=args.resident_slots, margin=1, team=args.resident_team),
               "throughput": dict(workers=8, slots=32, team=0, margin=2),
               "latency": dict(workers=args.serve_workers, slots=args.serve_slots, team=args.serve_team, margin=1)}
     setups = {k: v for k, v in setups.items() if k in args.serve_setups.split(",")}
@@ -777,7 +781,7 @@ def serve_leg(args, precision, log):
     def server(name, kw):
         # resident: frames admitted into one running kernel (ResidentServer);
         # the others: a launch per batch of queued frames (FitServer)
-        return dao_serve.ResidentServer(**kw) if name == "resident" else dao_serve.FitServer(**kw)
+        return dao_serve.ResidentServer(**kw) if name.startswith("resident") else dao_serve.FitServer(**kw)
 
     for name, su in setups.items():
         srv_kw = dict(base, **su)

@@ -43,9 +43,14 @@ typedef struct daoResident daoResident;
  * compute unit but 8, which stay free for other kernels -- a resident
  * workgroup holds all of its CU's registers), doFit with at most
  * max_iterations sweeps after the first (2000 in the reference), workgroups
- * leave after idle_seconds without a request.  NULL (daoGetLastError) for an
- * error model other than MLE. */
-daoResident *daoResidentCreate(daoBatch *b, int workgroups, int max_iterations, double idle_seconds);
+ * leave once idle for idle_seconds.  team_peaks = 0: each request is fitted
+ * by one workgroup (the most frames per second); > 0: a request of n start
+ * peaks by a team of min(64, ceil(n / team_peaks)) workgroups on as many CUs,
+ * fewer when workgroups are already given to requests not yet collected (at
+ * least one): lower latency per frame while the device has room; results
+ * identical for every team size.  NULL (daoGetLastError) for
+ * an error model other than MLE. */
+daoResident *daoResidentCreate(daoBatch *b, int workgroups, int max_iterations, double idle_seconds, int team_peaks);
 /* Stops the kernel once the outstanding requests are done, frees everything
  * (not b). */
 void daoResidentDestroy(daoResident *r);

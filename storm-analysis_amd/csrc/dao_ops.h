@@ -2409,22 +2409,15 @@ __device__ DAO_TEAM_COLD void team_transition(const FitArgs& a, const Frame& fr,
   __syncthreads();
 }
 
-// Workgroup b of the launch belongs to frame team_block_frame[b]; its waves
-// use seats team_block_seat[b] + wave.  Dynamic LDS as k_fit_frames.
+// All of doFit for frame f by one workgroup of its team: the frame's seats
+// are [s0, s1), this workgroup's waves use seats seat0 + wave.  Counter
+// increments accumulate in sh.cnt.
 template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
-__global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
+__device__ void team_fit(const FitArgs& a, int f, const Frame& fr, int s0, int s1, int seat0, TeamShared& sh,
+                         double* slice) {
   static_assert(NW <= kTeamSeats, "seats per workgroup");
-  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
-  __shared__ TeamShared sh;
-  extern __shared__ double lds_dyn[];
-  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
-  const int f = a.team_block_frame[blockIdx.x];
-  const Frame fr = a.frames[f];
   TeamCtl* const ctl = a.team_ctl + f;
-  const int s0 = a.team_seat0[f], s1 = a.team_seat0[f + 1];
-  const int seat = a.team_block_seat[blockIdx.x] + (threadIdx.x >> 6);
-  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
-  __syncthreads();
+  const int seat = seat0 + (threadIdx.x >> 6);
   int g = 0;  // the last generation this workgroup ran (0: none)
   for (;;) {
     if (threadIdx.x < kWave) {
@@ -2457,6 +2450,22 @@ __global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
     __syncthreads();
   }
   __syncthreads();
+}
+
+// Workgroup b of the launch belongs to frame team_block_frame[b]; its waves
+// use seats team_block_seat[b] + wave.  Dynamic LDS as k_fit_frames.
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
+__global__ __launch_bounds__(NW * 64) void k_fit_team(FitArgs a) {
+  constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
+  __shared__ TeamShared sh;
+  extern __shared__ double lds_dyn[];
+  double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
+  const int f = a.team_block_frame[blockIdx.x];
+  const Frame fr = a.frames[f];
+  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  __syncthreads();
+  team_fit<PPL, SHAPE, NW, MP, EM>(a, f, fr, a.team_seat0[f], a.team_seat0[f + 1], a.team_block_seat[blockIdx.x], sh,
+                                   slice);
   if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
     atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
 }

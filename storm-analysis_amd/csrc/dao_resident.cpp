@@ -58,13 +58,21 @@ struct daoResident {
   std::vector<int*> out_status;
   std::vector<int> state;
   std::vector<long long> seq;
+  std::vector<int> team_of;  // workgroups the slot's request was given
+  int members = 0;           // workgroups given to requests not yet collected
   long long* d_head = nullptr;  // [0]: tickets taken, [1]: clock of the last take, [2]: frames in flight
   double* d_rows = nullptr;
   long long* d_pix_cap = nullptr;
+  // team requests: per-slot team state, scheduling control and seats
+  int team_peaks = 0;  // > 0: a frame of n peaks is fitted by ceil(n / team_peaks) workgroups
+  dao::ResTeamSlot* d_team = nullptr;
+  dao::TeamCtl* d_team_ctl = nullptr;
+  int* d_team_cur = nullptr;
   hipStream_t st = nullptr;
   hipEvent_t ev = nullptr;
   bool running = false;
-  long long tail = 0;
+  long long tail = 0;      // ring entries published
+  long long next_seq = 0;  // requests published
   long long launches = 0;
   int blocks = 0;
   int max_iterations = 2000;
@@ -84,6 +92,8 @@ struct daoResident {
     }
     Engine& e = eng();
     dao::FitArgs a = e.fitter_args(max_iterations);
+    a.team_ctl = d_team_ctl;
+    a.team_cur = d_team_cur;
     dao::ResArgs ra;
     ra.q = device_view(q);
     ra.ring = device_view(ring);
@@ -91,15 +101,19 @@ struct daoResident {
     ra.head = d_head;
     ra.last = d_head + 1;
     ra.busy = d_head + 2;
+    ra.team = d_team;
     ra.rows_dev = d_rows;
     ra.pix_cap = d_pix_cap;
     ra.nslots = (int)state.size();
     ra.qcap = qcap;
     ra.idle_ticks = idle_ticks;
-    if (a.m.prec == dao::PREC_MIXED) {
-      dao::launch_resident_mixed(a, ra, blocks, st);
+    const bool mixed = a.m.prec == dao::PREC_MIXED;
+    if (team_peaks > 0) {
+      if (mixed) dao::launch_resident_team_mixed(a, ra, blocks, st);
+      else dao::launch_resident_team_fp64(a, ra, blocks, st);
     } else {
-      dao::launch_resident_fp64(a, ra, blocks, st);
+      if (mixed) dao::launch_resident_mixed(a, ra, blocks, st);
+      else dao::launch_resident_fp64(a, ra, blocks, st);
     }
     DAO_CHECK(hipGetLastError());
     DAO_CHECK(hipEventRecord(ev, st));
@@ -121,6 +135,9 @@ struct daoResident {
     if (d_head) (void)hipFree(d_head);
     if (d_rows) (void)hipFree(d_rows);
     if (d_pix_cap) (void)hipFree(d_pix_cap);
+    if (d_team) (void)hipFree(d_team);
+    if (d_team_ctl) (void)hipFree(d_team_ctl);
+    if (d_team_cur) (void)hipFree(d_team_cur);
     if (ev) (void)hipEventDestroy(ev);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -128,17 +145,19 @@ struct daoResident {
 
 extern "C" {
 
-daoResident* daoResidentCreate(daoBatch* b, int workgroups, int max_iterations, double idle_seconds) {
+daoResident* daoResidentCreate(daoBatch* b, int workgroups, int max_iterations, double idle_seconds, int team_peaks) {
   daoResident* r = nullptr;
   const int rc = dao::guarded([&] {
-    if (b == nullptr || max_iterations < 0 || !(idle_seconds > 0.0)) dao::fail(dao::kErrArg, "bad resident fitter arguments");
+    if (b == nullptr || max_iterations < 0 || !(idle_seconds > 0.0) || team_peaks < 0)
+      dao::fail(dao::kErrArg, "bad resident fitter arguments");
     Engine& e = *b->eng;
     if (e.m.errm != dao::EM_MLE) dao::fail(dao::kErrArg, "the resident fitter supports the MLE error model only");
     r = new daoResident();
     try {
       r->b = b;
       const int F = e.F;
-      r->qcap = 2 * F;
+      r->qcap = F * dao::kMaxTeam;  // every slot's request, each with up to kMaxTeam tickets
+      r->team_peaks = team_peaks;
       r->q = pinned<dao::ResQueue>(1);
       r->ring = pinned<int>(r->qcap);
       r->slots = pinned<dao::ResSlot>(F);
@@ -147,6 +166,7 @@ daoResident* daoResidentCreate(daoBatch* b, int workgroups, int max_iterations, 
       r->out_status.assign(F, nullptr);
       r->state.assign(F, kFree);
       r->seq.assign(F, 0);
+      r->team_of.assign(F, 0);
       for (int f = 0; f < F; f++) {
         r->out_rows[f] = pinned<double>(9 * (size_t)e.fr[f].cap);
         r->out_status[f] = pinned<int>(e.fr[f].cap);
@@ -155,6 +175,15 @@ daoResident* daoResidentCreate(daoBatch* b, int workgroups, int max_iterations, 
       DAO_CHECK(hipMemset(r->d_head, 0, 3 * sizeof(long long)));
       DAO_CHECK(hipMalloc((void**)&r->d_rows, sizeof(double) * 7 * (size_t)std::max(1LL, e.cap)));
       DAO_CHECK(hipMalloc((void**)&r->d_pix_cap, sizeof(long long) * F));
+      if (team_peaks > 0) {
+        DAO_CHECK(hipMalloc((void**)&r->d_team, sizeof(dao::ResTeamSlot) * F));
+        DAO_CHECK(hipMemset(r->d_team, 0, sizeof(dao::ResTeamSlot) * F));
+        DAO_CHECK(hipMalloc((void**)&r->d_team_ctl, sizeof(dao::TeamCtl) * F));
+        DAO_CHECK(hipMemset(r->d_team_ctl, 0, sizeof(dao::TeamCtl) * F));
+        const size_t seats = (size_t)F * dao::kMaxTeam * dao::kTeamSeats;
+        DAO_CHECK(hipMalloc((void**)&r->d_team_cur, sizeof(int) * seats));
+        DAO_CHECK(hipMemset(r->d_team_cur, 0, sizeof(int) * seats));
+      }
       DAO_CHECK(hipMemcpy(r->d_pix_cap, e.pix_cap.data(), sizeof(long long) * F, hipMemcpyHostToDevice));
       DAO_CHECK(hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking));
       DAO_CHECK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
@@ -216,9 +245,20 @@ int daoResidentSubmit(daoResident* r, int slot, int width, int height, const dou
       rw = d + 2 * npx;
     }
     dao::ResSlot& s = r->slots[slot];
-    const long long t = ++r->tail;  // ticket t is published as ring entry t - 1
+    const long long t = ++r->next_seq;
+    // a team of ceil(n / team_peaks) workgroups, at most as many as are not
+    // already given to requests in flight: large teams while the device has
+    // room (latency), one workgroup each when it is busy (throughput)
+    int team = 1;
+    if (r->team_peaks > 0) {
+      const int want = std::min(dao::kMaxTeam, (n_rows + r->team_peaks - 1) / r->team_peaks);
+      team = std::max(1, std::min(want, r->blocks - r->members));
+    }
+    r->team_of[slot] = team;
+    r->members += team;
     s.done = 0;
     s.seq = t;
+    s.team = team;
     s.img = img;
     s.bg = bg;
     s.rows = n_rows > 0 ? rw : img;
@@ -230,8 +270,11 @@ int daoResidentSubmit(daoResident* r, int slot, int width, int height, const dou
     s.fmt = fmt;
     s.rqe = 1.0;
     s.scm = 0.0;
-    __atomic_store_n(&r->ring[(t - 1) % r->qcap], slot, __ATOMIC_RELAXED);
-    __atomic_store_n(&r->q->tail, t, __ATOMIC_RELEASE);
+    // one ticket per team member, member 0 (which loads the frame) first
+    for (int m = 0; m < team; m++)
+      __atomic_store_n(&r->ring[(r->tail + m) % r->qcap], slot | (m << dao::kResMemberShift), __ATOMIC_RELAXED);
+    r->tail += team;
+    __atomic_store_n(&r->q->tail, r->tail, __ATOMIC_RELEASE);
     r->state[slot] = kQueued;
     r->seq[slot] = t;
     if (ticket) *ticket = t;
@@ -284,6 +327,8 @@ int daoResidentResult(daoResident* r, int slot, double* out, int32_t* status, in
     }
     r->state[slot] = kFree;
     r->seq[slot] = 0;
+    r->members -= r->team_of[slot];
+    r->team_of[slot] = 0;
     if (s.err) {
       n = dao::kErrFrame;
       dao::set_last_error(dao::kErrFrame,
@@ -302,7 +347,7 @@ int daoResidentResult(daoResident* r, int slot, double* out, int32_t* status, in
 
 void daoResidentStats(daoResident* r, int64_t* out2) {
   std::lock_guard<std::mutex> lock(r->mu);
-  out2[0] = r->tail;
+  out2[0] = r->next_seq;
   out2[1] = r->launches;
 }
 

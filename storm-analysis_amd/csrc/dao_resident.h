@@ -27,6 +27,9 @@ struct ResSlot {
   double* out;         // mapped host memory: n x 9 {x, y, z, height, background, xsigma, ysigma, error, iterations}
   int* status;         // mapped host memory: n
   int W, H, n, fmt;
+  int team;            // workgroups fitting the frame (1: one workgroup, the frame-per-workgroup
+                       // fitter; more: members 0..team-1 of a team, one ticket each)
+  int pad0;
   double rqe, scm;     // uniform calibration
   // result
   int counters[C_COUNT];
@@ -39,6 +42,20 @@ struct ResSlot {
   long long done;            // seq once the result is complete
 };
 
+// Device-side state of a team request in slot f: member 0 loads the frame
+// and sets `ready` (= the request's seq) once the others may start; the
+// member that leaves last (`left` reaches the team size) writes the result.
+struct ResTeamSlot {
+  long long ready;
+  int left;
+  int err;              // the load refused the request (as ResSlot::err)
+  int W, H, nfit, pad;  // the frame as loaded
+  int cnt[C_COUNT];     // counter increments of the load and of every member
+};
+
+// A ticket in the ring: slot | member << 16.
+constexpr int kResMemberShift = 16;
+
 struct ResQueue {
   long long tail;  // tickets published
   long long stop;  // non-zero: workgroups leave once no ticket is waiting
@@ -46,11 +63,12 @@ struct ResQueue {
 
 struct ResArgs {
   ResQueue* q;         // mapped host memory
-  const int* ring;     // mapped host memory: slot of ticket t at ring[t % qcap]
+  const int* ring;     // mapped host memory: ticket t at ring[t % qcap] (slot | member << 16)
   ResSlot* slots;      // mapped host memory, one per slot of the batch
   long long* head;     // device: tickets taken
   long long* last;     // device: wall clock when a ticket was last taken
   long long* busy;     // device: workgroups fitting a frame
+  ResTeamSlot* team;   // device: one per slot
   double* rows_dev;    // device: start rows of slot f at rows_dev + 7 * peak0
   const long long* pix_cap;  // device: pixel capacity of each slot
   int nslots;
@@ -59,9 +77,12 @@ struct ResArgs {
                          // for this long
 };
 
-// resident fitter launchers (dao_resident_fp64.hip, dao_resident_mixed.hip):
-// `blocks` workgroups that take tickets until none arrives for idle_ticks
+// resident fitter launchers (dao_resident_{fp64,mixed}{,_team}.hip):
+// `blocks` workgroups that take tickets until the server is idle; every
+// request fitted by one workgroup, or (team) by a team of workgroups
 void launch_resident_fp64(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st);
 void launch_resident_mixed(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st);
+void launch_resident_team_fp64(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st);
+void launch_resident_team_mixed(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st);
 
 }  // namespace dao

@@ -7,7 +7,7 @@ namespace dao {
 void launch_resident_fp64(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st) {
   DAO_PPL_SWITCH(a.m.nroi, DAO_SHAPE_SWITCH(a.m.shape, ({
                    constexpr int NW = PPL <= 8 ? 8 : 4;
-                   launch_resident_t<PPL, SHP, NW, false, EM_MLE>(a, r, blocks, st);
+                   launch_resident_t<PPL, SHP, NW, false, EM_MLE, false>(a, r, blocks, st);
                  })));
 }
 

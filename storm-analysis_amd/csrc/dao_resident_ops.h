@@ -38,7 +38,8 @@ __device__ inline void sys_st(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Thread 0: the slot of the next ticket, or -1 when the workgroup should
+// Thread 0: the next ticket (slot | member << kResMemberShift), or -1 when
+// the workgroup should
 // leave: the host asked to stop, or no frame is being fitted and no
 // workgroup has taken a ticket for idle_ticks (the whole kernel then leaves
 // at about the same time, and the host starts it again with the next
@@ -64,8 +65,9 @@ __device__ inline int res_take(const ResArgs& r) {
                                                  __HIP_MEMORY_SCOPE_AGENT)) {
           __hip_atomic_fetch_add(r.busy, 1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           __hip_atomic_fetch_max(r.last, wall_ticks(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          const int f = sys_ld(&r.ring[h % r.qcap]);
-          if (f >= 0 && f < r.nslots) return f;
+          const int t = sys_ld(&r.ring[h % r.qcap]);
+          const int f = t & ((1 << kResMemberShift) - 1);
+          if (t >= 0 && f < r.nslots) return t;
           __hip_atomic_fetch_add(r.busy, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         continue;
@@ -90,10 +92,70 @@ struct ResReq {
   const double* rows;
   double* out;
   int* status;
-  int W, H, n, fmt;
+  int W, H, n, fmt, team;
   double rqe, scm;
   long long t_take, t_load;
 };
+
+// Thread 0: the request fields of slot rs.
+__device__ inline void res_read(const ResSlot* rs, ResReq& q) {
+  q.seq = sys_ld(&rs->seq);
+  q.img = sys_ld(&rs->img);
+  q.bg = sys_ld(&rs->bg);
+  q.rows = sys_ld(&rs->rows);
+  q.out = sys_ld(&rs->out);
+  q.status = sys_ld(&rs->status);
+  q.W = sys_ld(&rs->W);
+  q.H = sys_ld(&rs->H);
+  q.n = sys_ld(&rs->n);
+  q.fmt = sys_ld(&rs->fmt);
+  q.team = sys_ld(&rs->team);
+  q.rqe = sys_ld(&rs->rqe);
+  q.scm = sys_ld(&rs->scm);
+  q.t_take = wall_ticks();
+  q.t_load = q.t_take;
+}
+
+// The n peaks of frame fr as k_pack_peaks' rows into the request's mapped
+// host memory (records read through M: device-coherent after a team fit).
+template <class M>
+__device__ void res_pack(const FitArgs& a, const Frame& fr, int n, const ResReq& q) {
+  const Model& m = a.m;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const PeakRec* rec = a.pk.rec + fr.peak0 + k;
+    double* o = q.out + 9LL * k;
+    sys_st(o + 0, M::ld(&rec->prm[P_XC]) + m.xoff);
+    sys_st(o + 1, M::ld(&rec->prm[P_YC]) + m.xoff);
+    sys_st(o + 2, M::ld(&rec->prm[P_ZC]) + m.zoff);
+    sys_st(o + 3, M::ld(&rec->prm[P_HEIGHT]));
+    sys_st(o + 4, M::ld(&rec->prm[P_BG]));
+    sys_st(o + 5, sqrt(1.0 / (2.0 * M::ld(&rec->prm[P_XW]))));
+    sys_st(o + 6, sqrt(1.0 / (2.0 * M::ld(&rec->prm[P_YW]))));
+    sys_st(o + 7, M::ld(&rec->err));
+    sys_st(o + 8, (double)M::ld(&rec->iter));
+    sys_st(q.status + k, M::ld(&rec->status));
+  }
+}
+
+// Completes the request of slot f: counters (cnt[C_COUNT], in LDS or read
+// through M), peak count, sweeps, error code, then `done` = seq last.
+template <class M>
+__device__ void res_publish(const ResArgs& r, int f, const ResReq& q, const int* cnt, int nfit, int sweeps, int err) {
+  ResSlot* const rs = r.slots + f;
+  if (threadIdx.x < C_COUNT) sys_st(&rs->counters[threadIdx.x], M::ld(&cnt[threadIdx.x]));
+  // every thread's result stores complete before thread 0 publishes
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sys_st(&rs->nfit, nfit);
+    sys_st(&rs->sweeps, sweeps);
+    sys_st(&rs->err, err);
+    sys_st(&rs->t_take, q.t_take);
+    sys_st(&rs->t_load, q.t_load);
+    sys_st(&rs->t_done, wall_ticks());
+    __hip_atomic_store(&rs->done, q.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
 
 __device__ inline int4 res_box(int x0, int y0, int roi, const Frame& fr) {
   int4 b;
@@ -114,20 +176,7 @@ __device__ __noinline__ Frame res_load(const FitArgs& a, const ResArgs& r, int f
   const Model& m = a.m;
   ResSlot* const rs = r.slots + f;
   if (threadIdx.x == 0) {
-    q.seq = sys_ld(&rs->seq);
-    q.img = sys_ld(&rs->img);
-    q.bg = sys_ld(&rs->bg);
-    q.rows = sys_ld(&rs->rows);
-    q.out = sys_ld(&rs->out);
-    q.status = sys_ld(&rs->status);
-    q.W = sys_ld(&rs->W);
-    q.H = sys_ld(&rs->H);
-    q.n = sys_ld(&rs->n);
-    q.fmt = sys_ld(&rs->fmt);
-    q.rqe = sys_ld(&rs->rqe);
-    q.scm = sys_ld(&rs->scm);
-    q.t_take = wall_ticks();
-    q.t_load = q.t_take;
+    res_read(rs, q);
     sh.failed = 0;
     a.frame_err[f] = 0;
   }
@@ -203,81 +252,152 @@ __device__ __noinline__ Frame res_load(const FitArgs& a, const ResArgs& r, int f
   return fr;
 }
 
+// A request fitted by this workgroup alone (the frame-per-workgroup fitter).
 template <int PPL, int SHAPE, int NW, bool MP, int EM>
 __device__ void res_frame(const FitArgs& a, const ResArgs& r, int f, FpwShared<NW>& sh, ResReq& q, double* slice) {
-  const Model& m = a.m;
-  ResSlot* const rs = r.slots + f;
   const Frame fr = res_load<PPL, SHAPE, NW, MP, EM>(a, r, f, sh, q, slice);
   __syncthreads();
   const int n = fr.nfit;
   const int sweeps = sh.failed ? 0 : fit_frame<PPL, SHAPE, NW, MP, EM>(a, f, fr, sh, slice);
-
-  // results (k_pack_peaks' rows)
-  const bool ok = sh.failed == 0;
-  if (ok) {
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      const PeakRec rec = a.pk.rec[fr.peak0 + k];
-      double* o = q.out + 9LL * k;
-      sys_st(o + 0, rec.prm[P_XC] + m.xoff);
-      sys_st(o + 1, rec.prm[P_YC] + m.xoff);
-      sys_st(o + 2, rec.prm[P_ZC] + m.zoff);
-      sys_st(o + 3, rec.prm[P_HEIGHT]);
-      sys_st(o + 4, rec.prm[P_BG]);
-      sys_st(o + 5, sqrt(1.0 / (2.0 * rec.prm[P_XW])));
-      sys_st(o + 6, sqrt(1.0 / (2.0 * rec.prm[P_YW])));
-      sys_st(o + 7, rec.err);
-      sys_st(o + 8, (double)rec.iter);
-      sys_st(q.status + k, rec.status);
-    }
-  }
-  if (threadIdx.x < C_COUNT) sys_st(&rs->counters[threadIdx.x], sh.cnt[threadIdx.x]);
-  // every thread's result stores complete before thread 0 publishes
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    sys_st(&rs->nfit, ok ? n : 0);
-    sys_st(&rs->sweeps, sweeps);
-    sys_st(&rs->err, ok ? 0 : (sh.failed >= 2 ? sh.failed : 1));
-    sys_st(&rs->t_take, q.t_take);
-    sys_st(&rs->t_load, q.t_load);
-    sys_st(&rs->t_done, wall_ticks());
-    __hip_atomic_store(&rs->done, q.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    __hip_atomic_fetch_add(r.busy, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  const int err = sh.failed == 0 ? 0 : (sh.failed >= 2 ? sh.failed : 1);
+  if (err == 0) res_pack<MemL1>(a, fr, n, q);
+  res_publish<MemL1>(r, f, q, sh.cnt, err == 0 ? n : 0, sweeps, err);
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(r.busy, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One member workgroup of a team request (ResSlot::team > 1).  Member 0
+// loads the frame as res_frame does, resets the team's scheduling state and
+// then lets the other members in (ResTeamSlot::ready); every member runs the
+// team fitter (team_fit: the frame's ops on any CU, in the reference's order
+// where ROIs overlap); the member that leaves last writes the result.  A
+// member never waits for one that has not started, except for member 0's
+// load, whose ticket precedes the others' in the ring.
 template <int PPL, int SHAPE, int NW, bool MP, int EM>
+__device__ void res_team(const FitArgs& a, const ResArgs& r, int f, int member, FpwShared<NW>& fsh,
+                         TeamShared& tsh, ResReq& q, double* slice, int* flag) {
+  ResTeamSlot* const ts = r.team + f;
+  if (member == 0) {
+    const Frame fr0 = res_load<PPL, SHAPE, NW, MP, EM>(a, r, f, fsh, q, slice);
+    __syncthreads();
+    const int s0 = f * kMaxTeam * kTeamSeats;
+    const int T = max(1, min(kMaxTeam, q.team));
+    for (int s = threadIdx.x; s < T * kTeamSeats; s += blockDim.x) MemDev::st(&a.team_cur[s0 + s], kIdle);
+    if (threadIdx.x < C_COUNT) MemDev::st(&ts->cnt[threadIdx.x], fsh.cnt[threadIdx.x]);
+    if (threadIdx.x == 0) {
+      TeamCtl* const ctl = a.team_ctl + f;
+      MemDev::st(&ctl->phase, 0);
+      MemDev::st(&ctl->lead, 0);
+      MemDev::st(&ctl->kind, 0);
+      MemDev::st(&ctl->n, 0);
+      MemDev::st(&ctl->next, 0);
+      MemDev::st(&ctl->done, 0);
+      MemDev::st(&ctl->esc, kNoEscape);
+      MemDev::st(&ctl->it, 0);
+      MemDev::st(&ctl->sweeps, 0);
+      MemDev::st(&ctl->nrun, 0);
+      MemDev::st(&a.frame_err[f], 0);
+      MemDev::st(&a.frame_sweeps[f], 0);
+      MemDev::st(&ts->left, 0);
+      MemDev::st(&ts->err, fsh.failed == 0 ? 0 : (fsh.failed >= 2 ? fsh.failed : 1));
+      MemDev::st(&ts->W, fr0.W);
+      MemDev::st(&ts->H, fr0.H);
+      MemDev::st(&ts->nfit, fr0.nfit);
+    }
+    dev_order();
+    __syncthreads();
+    if (threadIdx.x == 0) MemDev::st(&ts->ready, q.seq);
+  } else if (threadIdx.x == 0) {
+    res_read(r.slots + f, q);
+    // member 0 took its ticket first and is loading the frame; the wait is
+    // bounded all the same (a member that gives up fits nothing)
+    q.W = 0;
+    for (long long t0 = wall_ticks();;) {
+      if (MemDev::ld(&ts->ready) == q.seq) {
+        q.W = 1;
+        break;
+      }
+      if (wall_ticks() - t0 > 20 * r.idle_ticks) break;
+      __builtin_amdgcn_s_sleep(2);
+    }
+    dev_order();
+  }
+  __syncthreads();
+  const int T = max(1, min(kMaxTeam, q.team));
+  if (member == 0 || q.W == 1) {
+    Frame fr = a.frames[f];
+    fr.W = MemDev::ld(&ts->W);
+    fr.H = MemDev::ld(&ts->H);
+    fr.nfit = MemDev::ld(&ts->nfit);
+    if (uniform_i(MemDev::ld(&ts->err)) == 0) {
+      if (threadIdx.x < C_COUNT) tsh.cnt[threadIdx.x] = 0;
+      __syncthreads();
+      const int s0 = f * kMaxTeam * kTeamSeats;
+      team_fit<PPL, SHAPE, NW, MP, EM>(a, f, fr, s0, s0 + T * kTeamSeats, s0 + member * kTeamSeats, tsh, slice);
+      if (threadIdx.x < C_COUNT && tsh.cnt[threadIdx.x] != 0) MemDev::add(&ts->cnt[threadIdx.x], tsh.cnt[threadIdx.x]);
+    }
+  }
+  dev_order();
+  __syncthreads();
+  if (threadIdx.x == 0) *flag = MemDev::add(&ts->left, 1) == T - 1;
+  __syncthreads();
+  if (*flag) {
+    // the last member: every other member's stores were ordered before its
+    // increment of `left`
+    dev_order();
+    Frame fr = a.frames[f];
+    fr.W = MemDev::ld(&ts->W);
+    fr.H = MemDev::ld(&ts->H);
+    fr.nfit = MemDev::ld(&ts->nfit);
+    int err = uniform_i(MemDev::ld(&ts->err));
+    if (err == 0 && uniform_i(MemDev::ld(&a.frame_err[f])) != 0) err = 1;
+    if (err == 0) res_pack<MemDev>(a, fr, fr.nfit, q);
+    res_publish<MemDev>(r, f, q, ts->cnt, err == 0 ? fr.nfit : 0, uniform_i(MemDev::ld(&a.frame_sweeps[f])), err);
+  }
+  if (threadIdx.x == 0) __hip_atomic_fetch_add(r.busy, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// TEAM = false: every request is fitted by one workgroup (res_frame); TEAM =
+// true: every request by a team of ResSlot::team workgroups (res_team, also
+// for a team of one).  Two kernels rather than one with both paths: the
+// register allocation of either alone keeps the LM loop free of spills.
+template <int PPL, int SHAPE, int NW, bool MP, int EM, bool TEAM>
 __global__ __launch_bounds__(NW * 64) void k_resident(FitArgs a, ResArgs r) {
   constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
   // new-peak and first-error ops use the FP64 ROI cache over the same slice
   static_assert(fit_frames_slice<PPL, MP>() >= RoiLds<PPL>::kDoubles, "slice holds the FP64 ROI cache");
   __shared__ FpwShared<NW> sh;
+  __shared__ TeamShared tsh;
   __shared__ ResReq q;
-  __shared__ int take;
+  __shared__ int take, flag;
   extern __shared__ double lds_dyn[];
   double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
   for (;;) {
     if (threadIdx.x == 0) take = res_take(r);
     __syncthreads();
-    const int f = take;
+    const int t = take;
     __syncthreads();
-    if (f < 0) break;
-    res_frame<PPL, SHAPE, NW, MP, EM>(a, r, f, sh, q, slice);
+    if (t < 0) break;
+    const int f = t & ((1 << kResMemberShift) - 1), member = t >> kResMemberShift;
+    if constexpr (TEAM) {
+      res_team<PPL, SHAPE, NW, MP, EM>(a, r, f, member, sh, tsh, q, slice, &flag);
+    } else {
+      res_frame<PPL, SHAPE, NW, MP, EM>(a, r, f, sh, q, slice);
+    }
     __syncthreads();
   }
 }
 
-template <int PPL, int SHP, int NW, bool MP, int EM>
+template <int PPL, int SHP, int NW, bool MP, int EM, bool TEAM>
 void launch_resident_t(const FitArgs& a, const ResArgs& r, int blocks, hipStream_t st) {
   const size_t bytes = sizeof(double) * (size_t)NW * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;
   if (bytes > allowed) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resident<PPL, SHP, NW, MP, EM>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resident<PPL, SHP, NW, MP, EM, TEAM>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
       return;  // the launch below fails; the caller's error check reports it
     allowed = bytes;
   }
-  k_resident<PPL, SHP, NW, MP, EM><<<blocks, NW * 64, bytes, st>>>(a, r);
+  k_resident<PPL, SHP, NW, MP, EM, TEAM><<<blocks, NW * 64, bytes, st>>>(a, r);
 }
 
 }  // namespace dao

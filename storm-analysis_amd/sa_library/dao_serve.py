@@ -301,7 +301,7 @@ def loadResidentC():
         vp = ctypes.c_void_p
         i32p = ndpointer(dtype=numpy.int32)
         f64p = ndpointer(dtype=numpy.float64)
-        lib.daoResidentCreate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.daoResidentCreate.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
         lib.daoResidentCreate.restype = vp
         lib.daoResidentDestroy.argtypes = [vp]
         lib.daoResidentSubmit.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_int,
@@ -325,12 +325,14 @@ class ResidentServer(object):
     (new image, background, start peaks, doFit) and write its peaks to
     pinned host memory.  A request waits in a host queue only while every
     slot is taken; submitting is a few host memory writes, with no launch.
+    team > 0 fits a frame of n peaks with ceil(n / team) workgroups (at
+    most 64) for lower latency; 0 with one workgroup per frame.
     Frames share no state: each result equals the frame fitted alone
     (BatchFitter) bit for bit.  MLE only; the remaining keywords are
     BatchFitter's (model, precision, roi_size, wx_params, ...).
     """
     def __init__(self, slots=64, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None, max_iterations=2000,
-                 margin=-1, workgroups=0, idle_seconds=0.5, wait_limit=0, **fitter_kw):
+                 margin=-1, workgroups=0, idle_seconds=0.5, wait_limit=0, team=0, **fitter_kw):
         self.clib = loadResidentC()
         self.max_shape = tuple(max_shape)
         self.max_peaks = max_peaks
@@ -342,7 +344,8 @@ class ResidentServer(object):
         self.bf = dao_batch.BatchFitter([self.max_shape] * slots, [max_peaks] * slots, **fitter_kw)
         self.bf.setTuning(0, margin, 0)
         self.bf.setWaitLimit(wait_limit)
-        self.handle = self.clib.daoResidentCreate(self.bf.handle, workgroups, max_iterations, idle_seconds)
+        self.team = team
+        self.handle = self.clib.daoResidentCreate(self.bf.handle, workgroups, max_iterations, idle_seconds, team)
         if not self.handle:
             code, msg = lastLibraryError(self.clib)
             self.bf.cleanup()

@@ -10,6 +10,9 @@ frames admitted into a kernel that is already running.
   to 4000 emitters) match the oracle to the parity bar of test_gpu_parity:
   identical status flags and per-peak iteration counts, x/y/z within 1e-3
   px, height/background/widths within 1e-4 relative.
+- Teams: with team > 0 a frame of n peaks is fitted by ceil(n / team)
+  workgroups (the team fitter inside the resident kernel), with the same
+  results.
 - The kernel's workgroups leave when idle and the next request starts it
   again; bad requests fail alone.
 """
@@ -53,12 +56,13 @@ def assert_same(got, want, what):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("team", [0, 16])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_served_frames_equal_the_frames_fitted_alone(precision):
+def test_served_frames_equal_the_frames_fitted_alone(precision, team):
     frames = small_frames()
     bgs = [synth.z_background(fr.image.shape) for fr in frames]
     srv = dao_serve.ResidentServer(slots=3, max_shape=(160, 160), max_peaks=80, model="Z", precision=precision,
-                                   **ZKW)
+                                   team=team, **ZKW)
     try:
         reqs = [srv.submit(fr.image, bg, dao_batch.formatPeaksGaussianPSF(fr.peaks, "hdf5"))
                 for fr, bg in zip(frames, bgs)]
@@ -94,12 +98,13 @@ def test_device_inputs_give_the_host_inputs_result():
 
 
 @pytest.mark.gpu
-def test_served_z_frames_256_to_1024_match_the_oracle():
+@pytest.mark.parametrize("team", [0, 100])
+def test_served_z_frames_256_to_1024_match_the_oracle(team):
     rng = numpy.random.default_rng(synth.SEED_BASE + 61)
     spec = [(256, 200), (512, 1000), (1024, 4000), (256, 400), (512, 2000), (1024, 2500)]
     frames = [synth.frame_z(rng, size=s, n=n) for s, n in spec]
     bgs = [synth.z_background(fr.image.shape) for fr in frames]
-    srv = dao_serve.ResidentServer(slots=4, max_shape=(1024, 1024), max_peaks=4000, model="Z", **ZKW)
+    srv = dao_serve.ResidentServer(slots=4, max_shape=(1024, 1024), max_peaks=4000, model="Z", team=team, **ZKW)
     try:
         reqs = [srv.submit(fr.image, bg, dao_batch.formatPeaksGaussianPSF(fr.peaks, "hdf5"))
                 for fr, bg in zip(frames, bgs)]
