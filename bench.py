@@ -99,6 +99,8 @@ def parse():
                    help="serving set-ups to run: resident (frames admitted into a running kernel), "
                         "throughput, latency (a launch per batch)")
     p.add_argument("--resident-slots", type=int, default=64, help="frame slots of the resident server")
+    p.add_argument("--tune-sweeps", type=int, default=10,
+                   help="sweeps of each autotuner trial on the uneven batch (its first sweeps hold most of the work)")
     p.add_argument("--resident-team", type=int, default=64,
                    help="peaks per workgroup of the resident server's team set-up (a 512^2 frame of 1000 "
                         "emitters gets 16 workgroups)")
@@ -332,10 +334,11 @@ def uneven_leg(log, seed=20261209):
     wx, wy), sigma U(1.2, 2.0), ROI 18: 8 frames of 128^2, 8 of 256^2, 4 of
     512^2 at densities U(0.1, 3) emitters/um^2 (160 nm pixels) and one 2048^2
     frame at 0.2 /um^2; per-pixel offset N(100, 5), gain N(2, 0.1), read
-    variance lognormal(ln 4, 0.8).  The autotuner picks the reserved-box
-    margin and the team size (peaks per workgroup; 0 = one workgroup per
-    frame, which leaves the 2048^2 frame on one CU); FP64 and mixed precision
-    are timed with the pick.
+    variance lognormal(ln 4, 0.8).  The autotuner picks the reserved tile
+    (ROI 18 plus 0-2 pixels each side: 18^2, 20^2, 22^2) and the team size
+    (peaks per workgroup; 0 = one workgroup per frame, which leaves the
+    2048^2 frame on one CU) from trials of args.tune_sweeps sweeps; FP64 and
+    mixed precision are timed with the pick.
     """
     import numpy as np
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D, dao_tune
@@ -363,9 +366,13 @@ def uneven_leg(log, seed=20261209):
             bf.newPeaks([f.peaks for f in frames], "hdf5")
 
         if prec == "mixed":
-            best, trials = dao_tune.autotune(bf, prepare, waves=(8,), margins=(1, 2), teams=(0, 500, 250, 125),
-                                             log=log)
-            out["autotune"] = {"best": {"waves": best[0], "margin": best[1], "team": best[2]}, "trials": trials}
+            t_tune = time.perf_counter()
+            best, trials = dao_tune.autotune(bf, prepare, waves=(8,), tiles=(ROI, ROI + 2, ROI + 4),
+                                             teams=(0, 500, 250, 125), trial_sweeps=args.tune_sweeps, log=log)
+            out["autotune"] = {"best": {"waves": best[0], "margin": best[1], "tile": ROI + 2 * best[1],
+                                        "team": best[2]},
+                               "trials": trials, "trial_sweeps": args.tune_sweeps,
+                               "seconds": time.perf_counter() - t_tune}
         else:
             b = out["autotune"]["best"]
             bf.setTuning(b["waves"], b["margin"], b["team"])

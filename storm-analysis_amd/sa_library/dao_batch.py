@@ -173,6 +173,8 @@ class BatchFitter(object):
             m.z_min, m.z_max = min_z, max_z
         self.model = model
         self.error_model = error_model
+        self.roi_size = roi_size
+        self.precision = precision
         h = numpy.array([s[0] for s in self.shapes], dtype=numpy.int32)
         w = numpy.array([s[1] for s in self.shapes], dtype=numpy.int32)
         self.caps = numpy.asarray(peak_caps, dtype=numpy.int32)

@@ -1,26 +1,106 @@
 """
-Autotuner of the batched fitter: times doFit on the caller's own frames for
-each candidate (waves per workgroup, reserved-box margin, team size) and
-keeps the fastest.
+Autotuner of the batched fitter: times short fits of the caller's own
+frames for each candidate and keeps the fastest.  No counterpart in the
+reference, whose fitter has no launch shape to choose.
 
-Waves per workgroup is the number of peaks a frame's workgroup fits at once
-(fewer waves leave room for more frames per CU); the margin is the slack of
-each peak's reserved box around its ROI (a larger box lets the ROI move
-without a rollback but serializes more neighbours); the team size is the
-number of peaks per workgroup when several workgroups share a frame (0: one
-workgroup per frame), which lets a large frame use many CUs.  Results do not
-depend on the choice (to the parity bar), so tuning on the workload itself
-is safe.
+The knobs:
+- waves: the peaks a frame's workgroup fits at once (one wavefront per peak
+  turn); fewer waves leave room for more frames per CU;
+- tile: the side of the reserved box each peak turn holds, the ROI plus the
+  margin on each side (for ROI 18: 18^2 with no margin up to 24^2 with
+  three pixels); a larger tile lets a peak's ROI move without a rollback
+  but serializes more neighbouring turns.  A tile smaller than the ROI
+  cannot hold a turn and is not a candidate;
+- team: peaks per workgroup when several workgroups share a frame (0: one
+  workgroup per frame), which lets a large frame use many CUs.
+
+Results do not depend on the choice (a turn that leaves its tile is rolled
+back exactly), so tuning on the workload itself is safe.  A trial fits the
+workload for `trial_sweeps` sweeps (the first sweeps hold most of the LM
+work; the long tail of a few slow peaks does not depend on the knobs),
+timed by the fit kernel's HIP events.  The pick is kept in a JSON cache
+keyed by the workload's signature -- model, error model, ROI, precision,
+each frame's size and its peak count rounded to a power of two, and the
+library build -- so a restarted server reuses it without trials.
 """
+import json
+import os
 import time
 
+from . import loadclib
 
-def autotune(bf, prepare, waves=(2, 4, 8), margins=(1, 2, 3), teams=(0,), max_iterations=2000, log=None):
+
+def workload_key(bf):
+    """Signature of the workload held by BatchFitter bf (see the module docstring)."""
+    sizes = {}
+    for f, (h, w) in enumerate(bf.shapes):
+        n = max(1, int(bf.getNFit(f)))
+        k = "%dx%d:%d" % (h, w, 1 << (n - 1).bit_length())
+        sizes[k] = sizes.get(k, 0) + 1
+    lib = loadclib.libraryPath("dao_fit")
+    try:
+        st = os.stat(lib)
+        build = "%d-%d" % (st.st_size, int(st.st_mtime))
+    except OSError:
+        build = "unknown"
+    return "|".join([bf.model, bf.error_model, "roi%d" % bf.roi_size, bf.precision, build] +
+                    ["%s*%d" % (k, v) for k, v in sorted(sizes.items())])
+
+
+def margins_for(tiles, roi_size):
+    """Reserved-box margins of the tile sides that can hold an ROI of roi_size."""
+    out = []
+    for t in tiles:
+        if t >= roi_size and (t - roi_size) % 2 == 0 and (t - roi_size) // 2 not in out:
+            out.append((t - roi_size) // 2)
+    return tuple(out)
+
+
+def _load_cache(path):
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(path, key, best):
+    data = _load_cache(path)
+    data[key] = {"waves": best[0], "margin": best[1], "team": best[2]}
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    tmp = path + ".tmp%d" % os.getpid()
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def autotune(bf, prepare, waves=(2, 4, 8), margins=None, teams=(0,), tiles=None, max_iterations=2000,
+             trial_sweeps=None, cache=None, log=None):
     """
     bf: a BatchFitter; prepare(): reloads the same images and start peaks into
-    it.  Returns (best, trials) where best = (waves, margin, team) is left set
-    on bf and trials lists {"waves", "margin", "team", "kernel_ms", "wall_ms"}.
+    it.  Candidates: waves x (margins, or the tile sides `tiles`, default
+    margins 1, 2, 3) x teams; each trial runs doFit with trial_sweeps sweeps
+    (None: max_iterations, the whole fit).  With `cache` (a JSON file path)
+    a pick stored for the same workload signature is used without trials,
+    and a new pick is stored.  Returns (best, trials): best = (waves, margin,
+    team), left set on bf; trials lists {"waves", "margin", "tile", "team",
+    "kernel_ms", "wall_ms"} (empty when the cache answered).
     """
+    key = workload_key(bf) if cache else None
+    if cache:
+        hit = _load_cache(cache).get(key)
+        if hit is not None:
+            best = (int(hit["waves"]), int(hit["margin"]), int(hit["team"]))
+            bf.setTuning(*best)
+            if log is not None:
+                log("autotune: cached pick for this workload: waves %d margin %d team %d" % best)
+            return best, []
+    if margins is None:
+        margins = margins_for(tiles, bf.roi_size) if tiles is not None else (1, 2, 3)
+    if not margins:
+        raise ValueError("no tile can hold the ROI")
+    sweeps = max_iterations if trial_sweeps is None else trial_sweeps
     trials = []
     bf.setTiming(True)
     for t in teams:
@@ -30,12 +110,17 @@ def autotune(bf, prepare, waves=(2, 4, 8), margins=(1, 2, 3), teams=(0,), max_it
                 prepare()
                 s0 = bf.stats()
                 t0 = time.perf_counter()
-                bf.doFit(max_iterations)
+                bf.doFit(sweeps)
                 wall = (time.perf_counter() - t0) * 1e3
                 ms = bf.stats()["lm_ms"] - s0["lm_ms"]
-                trials.append({"waves": w, "margin": m, "team": t, "kernel_ms": ms, "wall_ms": wall})
+                trials.append({"waves": w, "margin": m, "tile": bf.roi_size + 2 * m, "team": t, "kernel_ms": ms,
+                               "wall_ms": wall})
                 if log is not None:
-                    log("autotune: waves %d margin %d team %d: fit kernel %.1f ms" % (w, m, t, ms))
-    best = min(trials, key=lambda r: r["kernel_ms"])
-    bf.setTuning(best["waves"], best["margin"], best["team"])
-    return (best["waves"], best["margin"], best["team"]), trials
+                    log("autotune: waves %d tile %d^2 team %d: fit kernel %.1f ms (%d sweeps)" %
+                        (w, bf.roi_size + 2 * m, t, ms, sweeps))
+    b = min(trials, key=lambda r: r["kernel_ms"])
+    best = (b["waves"], b["margin"], b["team"])
+    bf.setTuning(*best)
+    if cache:
+        _store_cache(cache, key, best)
+    return best, trials
