@@ -208,6 +208,16 @@ struct FitArgs {
   const int* team_seat0;        // [F + 1]: frame f's seats are [team_seat0[f], team_seat0[f + 1])
 };
 
+// One frame's share of a k_add_peaks launch: n start rows (device memory,
+// daoNewPeaks layout) become peaks first .. first + n - 1 of frame `frame`.
+struct AddPeaksJob {
+  const double* rows;
+  int frame;
+  int n;
+  int first;
+  int pad;
+};
+
 // The multiplane fitter (k_fit_mp): frame-sets of nch channels, channel c of
 // set k in frame set_frame0[k] + c; peak i of a set is slot peak0 + i of each
 // channel's frame.  cpl: the coupling table in device memory, in this order:

@@ -684,57 +684,49 @@ void Engine::new_peaks(int f, const double* rows, int fmt, int n) {
   ensure_frame_cap(f, fr[f].nfit + n);
   double* d_rows = stage(sizeof(double) * (size_t)n * width);
   DAO_CHECK(hipMemcpyAsync(d_rows, rows, sizeof(double) * (size_t)n * width, hipMemcpyHostToDevice, st));
-  int first = fr[f].peak0 + fr[f].nfit;
-  launch_iota(sc.op_peak, n, first, st);
-  sc.nops = n;
-  launch_fill_i32(sc.frame_bad, F, 0, st);
-  launch_box(OP_NEWPEAK, sc, pk, d_fr, m, nullptr, d_rows, fmt, first, st);
-  run_ops(OP_NEWPEAK, d_rows, fmt, first);
-  launch_commit(sc, pk, d_counters, st);
-  fr[f].nfit += n;
-  upload_frames();
-  // initial errors, in peak order
-  launch_box(OP_PEAKERROR, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
-  run_ops(OP_PEAKERROR, nullptr, 0, 0);
-  launch_commit(sc, pk, d_counters, st);
-  sync();
+  add_peaks({{d_rows, f, n, fr[f].peak0 + fr[f].nfit, 0}}, fmt);
 }
 
-// Batch form: new peaks for several frames in one executor pass.  rows_dev[k]
-// are device pointers; all frames use the same format.
+// Batch form: new peaks for several frames in one launch.  rows_dev[k] are
+// device pointers; all frames use the same format.
 void Engine::new_peaks_multi(const std::vector<int>& frames, const std::vector<const double*>& rows_dev,
                              const std::vector<int>& counts, int fmt) {
-  int width = fmt == PF_TEXT ? 7 : 4;
-  long long total = 0;
-  for (size_t k = 0; k < frames.size(); k++) total += counts[k];
-  if (total == 0) return;
-  // concatenate rows in op order (frames ascending)
-  std::vector<size_t> order(frames.size());
-  for (size_t k = 0; k < order.size(); k++) order[k] = k;
-  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return frames[a] < frames[b]; });
-  double* d_rows = stage(sizeof(double) * (size_t)total * width);
-  int* ops = istage(sizeof(int) * (size_t)total);
-  long long at = 0;
-  for (size_t k : order) {
-    int f = frames[k], n = counts[k];
+  std::vector<AddPeaksJob> jobs;
+  for (size_t k = 0; k < frames.size(); k++) {
+    const int f = frames[k], n = counts[k];
     if (n == 0) continue;
     if (fr[f].nfit + n > fr[f].cap) fail(kErrCapacity, "more peaks than the frame's capacity");
-    DAO_CHECK(hipMemcpyAsync(d_rows + at * width, rows_dev[k], sizeof(double) * (size_t)n * width,
-                             hipMemcpyDeviceToDevice, st));
-    launch_iota(ops + at, n, fr[f].peak0 + fr[f].nfit, st);
-    at += n;
+    jobs.push_back({rows_dev[k], f, n, fr[f].peak0 + fr[f].nfit, 0});
   }
-  DAO_CHECK(hipMemcpyAsync(sc.op_peak, ops, sizeof(int) * total, hipMemcpyDeviceToDevice, st));
-  sc.nops = (int)total;
-  launch_fill_i32(sc.frame_bad, F, 0, st);
-  launch_box(OP_NEWPEAK, sc, pk, d_fr, m, nullptr, d_rows, fmt, 0, st);
-  run_ops(OP_NEWPEAK, d_rows, fmt, 0);
-  launch_commit(sc, pk, d_counters, st);
-  for (size_t k = 0; k < frames.size(); k++) fr[frames[k]].nfit += counts[k];
+  add_peaks(jobs, fmt);
+}
+
+// daoNewPeaks' image work for each job (at most one per frame): k_add_peaks,
+// one workgroup per frame, the peaks added in index order where their ROIs
+// overlap and then their first errors, as the reference does one frame at a
+// time.  The frames' peak counts grow by the jobs' row counts.
+void Engine::add_peaks(const std::vector<AddPeaksJob>& jobs, int fmt) {
+  if (jobs.empty()) return;
+  FitArgs a = fitter_args(0);
+  // the caller's wait limit (daoBatchSetWaitLimit) is the fit's; adding
+  // peaks keeps the default bound
+  a.spin_limit = kSpinLimit;
+  const size_t bytes = sizeof(AddPeaksJob) * jobs.size();
+  AddPeaksJob* h = reinterpret_cast<AddPeaksJob*>(pin(bytes + sizeof(int) * (size_t)F));
+  std::memcpy(h, jobs.data(), bytes);
+  AddPeaksJob* d = reinterpret_cast<AddPeaksJob*>(istage(bytes));
+  DAO_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
+  launch_fill_i32(d_err, F, 0, st);
+  // more frames than CUs: four waves per workgroup, two frames per CU
+  launch_add_peaks(a, d, (int)jobs.size(), fmt, (int)jobs.size() > num_cus ? 4 : 0, st);
+  DAO_CHECK(hipGetLastError());
+  int* herr = reinterpret_cast<int*>(reinterpret_cast<char*>(h) + bytes);
+  DAO_CHECK(hipMemcpyAsync(herr, d_err, sizeof(int) * F, hipMemcpyDeviceToHost, st));
+  sync();
+  for (const AddPeaksJob& j : jobs)
+    if (herr[j.frame] != 0) fail(kErrSched, "adding peaks: a scheduler wait gave up");
+  for (const AddPeaksJob& j : jobs) fr[j.frame].nfit += j.n;
   upload_frames();
-  launch_box(OP_PEAKERROR, sc, pk, d_fr, m, nullptr, nullptr, 0, 0, st);
-  run_ops(OP_PEAKERROR, nullptr, 0, 0);
-  launch_commit(sc, pk, d_counters, st);
 }
 
 // Stable compaction of frame f's live peaks to the slots listed in keep.

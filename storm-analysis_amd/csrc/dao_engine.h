@@ -106,6 +106,8 @@ void launch_select(const Peaks& pk, const Frame* frames, long long nslots, int m
 void launch_fill_i32(int* p, long long n, int v, hipStream_t st);
 void launch_iota(int* p, int n, int base, hipStream_t st);
 void launch_fit_frames(const FitArgs& a, int nframes, hipStream_t st);
+// k_add_peaks over njobs workgroups of nw waves (0: the kernel's maximum)
+void launch_add_peaks(const FitArgs& a, const AddPeaksJob* jobs, int njobs, int fmt, int nw, hipStream_t st);
 void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st);
 void launch_fit_mp(const MpArgs& a, hipStream_t st);  // dao_mp.hip
 void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream_t st);
@@ -177,6 +179,7 @@ class Engine {
   void new_peaks(int f, const double* rows_host, int fmt, int n);
   void new_peaks_multi(const std::vector<int>& frames, const std::vector<const double*>& rows_dev,
                        const std::vector<int>& counts, int fmt);
+  void add_peaks(const std::vector<AddPeaksJob>& jobs, int fmt);
   void compact(int f, const std::vector<int32_t>& keep);
   void set_status(int f, const int32_t* status);
   // every frame at once, on the device (rows in frame order, as pack_all):

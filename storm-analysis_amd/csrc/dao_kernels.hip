@@ -794,6 +794,14 @@ void launch_team_init(TeamCtl* ctl, int nframes, int* cur, int nseats, hipStream
   if (n > 0) k_team_init<<<flat_grid(n), 256, 0, st>>>(ctl, nframes, cur, nseats);
 }
 
+void launch_add_peaks(const FitArgs& a, const AddPeaksJob* jobs, int njobs, int fmt, int nw, hipStream_t st) {
+  if (njobs == 0) return;
+  DAO_PPL_SWITCH(a.m.nroi, ({
+                   constexpr int NW = PPL <= 8 ? 8 : 4;
+                   launch_add_peaks_t<PPL, NW>(a, jobs, njobs, fmt, nw, st);
+                 }));
+}
+
 // k_fit_team over nblocks workgroups, FP64 or mixed precision (MLE only);
 // instantiations in dao_team_fp64.hip and dao_team_mixed.hip.
 void launch_fit_team(const FitArgs& a, int nblocks, hipStream_t st) {

@@ -2070,6 +2070,63 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;
 }
 
+// The ROI box at origin (x0, y0), clipped to the frame.
+__device__ inline int4 box_at(int x0, int y0, int roi, const Frame& fr) {
+  int4 b;
+  b.x = max(0, min(fr.W, x0));
+  b.y = max(0, min(fr.H, y0));
+  b.z = max(b.x, min(fr.W, x0 + roi));
+  b.w = max(b.y, min(fr.H, y0 + roi));
+  return b;
+}
+
+// daoNewPeaks' image work (dao_fit.c:1582-1766) for n start rows of frame f
+// by the calling workgroup: op k (box: the ROI at the floor of row k's
+// position) adds peak list[peak0 + k] from row k, in index order where ROIs
+// overlap (the ordered executor of the frame fitter), then every new peak's
+// first error is computed (dao_fit.c:1758-1763; box: its ROI).  Counter
+// increments accumulate in sh.cnt.
+template <int PPL, int NW>
+__device__ void add_peaks(const FitArgs& a, int f, const Frame& fr, int n, const double* rows, int fmt, int first,
+                          FpwShared<NW>& sh, double* slice) {
+  const Model& m = a.m;
+  const int wid = fmt == PF_TEXT ? 7 : 4;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int slot = fr.peak0 + k;
+    a.op_peak[slot] = first + k;
+    const double* row = rows + (long long)k * wid;
+    a.box[slot] = box_at((int)floor(row[0] - m.xoff), (int)floor(row[1] - m.xoff), m.roi, fr);
+  }
+  __syncthreads();
+  fpw_run<PPL, SH_2DFIXED, NW, false, -1, OP_NEWPEAK>(a, f, fr, n, a.op_peak, sh, slice, rows, fmt);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const PeakRec* p = a.pk.rec + first + k;
+    a.box[fr.peak0 + k] = box_at(p->xi, p->yi, m.roi, fr);
+  }
+  __syncthreads();
+  fpw_run<PPL, SH_2DFIXED, NW, false, -1, OP_PEAKERROR>(a, f, fr, n, a.op_peak, sh, slice);
+}
+
+// add_peaks for a batch of frames, one persistent workgroup per job (one
+// launch instead of an executor round per dependency level); dynamic LDS as
+// k_fit_frames (FP64 slices).
+template <int PPL, int NW>
+__global__ __launch_bounds__(NW * 64) void k_add_peaks(FitArgs a, const AddPeaksJob* jobs, int fmt) {
+  __shared__ FpwShared<NW> sh;
+  extern __shared__ double lds_dyn[];
+  double* const slice = lds_dyn + (threadIdx.x >> 6) * fit_frames_slice<PPL, false>();
+  const AddPeaksJob job = jobs[blockIdx.x];
+  const int f = job.frame;
+  const Frame fr = a.frames[f];
+  if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sh.failed = 0;
+  __syncthreads();
+  add_peaks<PPL, NW>(a, f, fr, job.n, job.rows, fmt, job.first, sh, slice);
+  __syncthreads();
+  if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
+    atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
+}
+
 // ---------------------------------------------------------------------------
 // frame-per-team fitter
 // ---------------------------------------------------------------------------
@@ -2510,6 +2567,21 @@ void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
     allowed = bytes;
   }
   k_fit_frames<PPL, SHP, NW, MP, EM><<<nframes, nw * 64, bytes, st>>>(a);
+}
+
+// Launch k_add_peaks<PPL, NW> over njobs workgroups of nw waves.
+template <int PPL, int NW>
+void launch_add_peaks_t(const FitArgs& a, const AddPeaksJob* jobs, int njobs, int fmt, int nw, hipStream_t st) {
+  nw = nw > 0 && nw < NW ? nw : NW;
+  const size_t bytes = sizeof(double) * (size_t)nw * fit_frames_slice<PPL, false>();
+  static size_t allowed = 64 * 1024;
+  if (bytes > allowed) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_add_peaks<PPL, NW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
+      return;
+    allowed = bytes;
+  }
+  k_add_peaks<PPL, NW><<<njobs, nw * 64, bytes, st>>>(a, jobs, fmt);
 }
 
 // Launch k_fit_team<PPL, SHP, NW, MP> over nblocks workgroups (the team

@@ -157,15 +157,6 @@ __device__ void res_publish(const ResArgs& r, int f, const ResReq& q, const int*
   }
 }
 
-__device__ inline int4 res_box(int x0, int y0, int roi, const Frame& fr) {
-  int4 b;
-  b.x = max(0, min(fr.W, x0));
-  b.y = max(0, min(fr.H, y0));
-  b.z = max(b.x, min(fr.W, x0 + roi));
-  b.w = max(b.y, min(fr.H, y0 + roi));
-  return b;
-}
-
 // Loads the ticket's request into q and the frame into slot f, adds its
 // start peaks and their first errors; returns the slot's frame record.  Out
 // of line: it runs once per frame, and its registers stay out of the LM
@@ -173,7 +164,6 @@ __device__ inline int4 res_box(int x0, int y0, int roi, const Frame& fr) {
 template <int PPL, int SHAPE, int NW, bool MP, int EM>
 __device__ __noinline__ Frame res_load(const FitArgs& a, const ResArgs& r, int f, FpwShared<NW>& sh, ResReq& q,
                                        double* slice) {
-  const Model& m = a.m;
   ResSlot* const rs = r.slots + f;
   if (threadIdx.x == 0) {
     res_read(rs, q);
@@ -232,22 +222,8 @@ __device__ __noinline__ Frame res_load(const FitArgs& a, const ResArgs& r, int f
   if (__any(bad) && lane_id() == 0) sh.failed = 3;
   __syncthreads();
   if (sh.failed) return fr;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const int slot = fr.peak0 + k;
-    a.op_peak[slot] = slot;
-    const double* row = rows + (long long)k * wid;
-    a.box[slot] = res_box((int)floor(row[0] - m.xoff), (int)floor(row[1] - m.xoff), m.roi, fr);
-  }
-  __syncthreads();
-  fpw_run<PPL, SHAPE, NW, MP, EM, OP_NEWPEAK>(a, f, fr, n, a.op_peak, sh, slice, rows, q.fmt);
+  add_peaks<PPL, NW>(a, f, fr, n, rows, q.fmt, fr.peak0, sh, slice);
   fr.nfit = n;
-  // every new peak's first error (dao_fit.c:1758-1763), box = its ROI
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const PeakRec* p = a.pk.rec + fr.peak0 + k;
-    a.box[fr.peak0 + k] = res_box(p->xi, p->yi, m.roi, fr);
-  }
-  __syncthreads();
-  fpw_run<PPL, SHAPE, NW, MP, EM, OP_PEAKERROR>(a, f, fr, n, a.op_peak, sh, slice);
   if (threadIdx.x == 0) q.t_load = wall_ticks();
   return fr;
 }
