@@ -95,7 +95,7 @@ def parse():
                    help="workers of the serving leg's latency set-up")
     p.add_argument("--serve-slots", type=int, default=32,
                    help="frame slots per worker in the serving leg's latency set-up")
-    p.add_argument("--serve-setups", default="resident,throughput,latency",
+    p.add_argument("--serve-setups", default="resident,latency",
                    help="serving set-ups to run: resident (frames admitted into a running kernel), "
                         "throughput, latency (a launch per batch)")
     p.add_argument("--resident-slots", type=int, default=64, help="frame slots of the resident server")
@@ -328,17 +328,17 @@ def result_rows(outs):
     return rows, st
 
 
-def uneven_leg(log, seed=20261209):
+def uneven_leg(log, tune_sweeps=10, seed=20261209):
     """
     Batched uneven frames in one launch with sCMOS weighting, '3d' model (free
     wx, wy), sigma U(1.2, 2.0), ROI 18: 8 frames of 128^2, 8 of 256^2, 4 of
     512^2 at densities U(0.1, 3) emitters/um^2 (160 nm pixels) and one 2048^2
     frame at 0.2 /um^2; per-pixel offset N(100, 5), gain N(2, 0.1), read
     variance lognormal(ln 4, 0.8).  The autotuner picks the reserved tile
-    (ROI 18 plus 0-2 pixels each side: 18^2, 20^2, 22^2) and the team size
-    (peaks per workgroup; 0 = one workgroup per frame, which leaves the
-    2048^2 frame on one CU) from trials of args.tune_sweeps sweeps; FP64 and
-    mixed precision are timed with the pick.
+    (ROI 18 plus 0-3 pixels each side: 18^2 to 24^2) on fits of
+    tune_sweeps sweeps, then the team size (peaks per workgroup; 0 = one
+    workgroup per frame, which leaves the 2048^2 frame on one CU) on whole
+    fits; FP64 and mixed precision are timed with the pick.
     """
     import numpy as np
     from storm_analysis_amd.sa_library import dao_batch, dao_fit_c as D, dao_tune
@@ -367,11 +367,11 @@ def uneven_leg(log, seed=20261209):
 
         if prec == "mixed":
             t_tune = time.perf_counter()
-            best, trials = dao_tune.autotune(bf, prepare, waves=(8,), tiles=(ROI, ROI + 2, ROI + 4),
-                                             teams=(0, 500, 250, 125), trial_sweeps=args.tune_sweeps, log=log)
+            best, trials = dao_tune.autotune(bf, prepare, waves=(8,), tiles=(ROI, ROI + 2, ROI + 4, ROI + 6),
+                                             teams=(0, 500, 250, 125), trial_sweeps=tune_sweeps, log=log)
             out["autotune"] = {"best": {"waves": best[0], "margin": best[1], "tile": ROI + 2 * best[1],
                                         "team": best[2]},
-                               "trials": trials, "trial_sweeps": args.tune_sweeps,
+                               "trials": trials, "trial_sweeps": tune_sweeps,
                                "seconds": time.perf_counter() - t_tune}
         else:
             b = out["autotune"]["best"]
@@ -900,7 +900,7 @@ def main():
 
     uneven = None
     if not args.no_uneven:
-        uneven = uneven_leg(log)
+        uneven = uneven_leg(log, tune_sweeps=args.tune_sweeps)
 
     dense = None
     if not args.no_dense:

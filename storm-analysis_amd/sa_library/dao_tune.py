@@ -1,6 +1,6 @@
 """
-Autotuner of the batched fitter: times short fits of the caller's own
-frames for each candidate and keeps the fastest.  No counterpart in the
+Autotuner of the batched fitter: times fits of the caller's own frames for
+each candidate and keeps the fastest.  No counterpart in the
 reference, whose fitter has no launch shape to choose.
 
 The knobs:
@@ -15,11 +15,17 @@ The knobs:
   workgroup per frame), which lets a large frame use many CUs.
 
 Results do not depend on the choice (a turn that leaves its tile is rolled
-back exactly), so tuning on the workload itself is safe.  A trial fits the
-workload for `trial_sweeps` sweeps (the first sweeps hold most of the LM
-work; the long tail of a few slow peaks does not depend on the knobs),
-timed by the fit kernel's HIP events.  The pick is kept in a JSON cache
-keyed by the workload's signature -- model, error model, ROI, precision,
+back exactly), so tuning on the workload itself is safe.  Trials are timed
+by the fit kernel's HIP events, in two stages when `trial_sweeps` is given:
+waves and tile are chosen on fits of `trial_sweeps` sweeps (the rollbacks a
+small tile causes and the occupancy of the waves show in the first sweeps),
+then the team size on whole fits at that choice: teams change the cost of
+every sweep of a large frame, and the long tail of sweeps with a few
+hundred peaks still running can hold most of the time (on an uneven sCMOS
+batch with a 2048^2 frame, 10-sweep trials ranked one workgroup per frame
+first, whose whole fit then took 1.7x as long as with teams).  Without
+`trial_sweeps` every candidate is a whole fit.  The pick is kept in a JSON
+cache keyed by the workload's signature -- model, error model, ROI, precision,
 each frame's size and its peak count rounded to a power of two, and the
 library build -- so a restarted server reuses it without trials.
 """
@@ -80,12 +86,14 @@ def autotune(bf, prepare, waves=(2, 4, 8), margins=None, teams=(0,), tiles=None,
     """
     bf: a BatchFitter; prepare(): reloads the same images and start peaks into
     it.  Candidates: waves x (margins, or the tile sides `tiles`, default
-    margins 1, 2, 3) x teams; each trial runs doFit with trial_sweeps sweeps
-    (None: max_iterations, the whole fit).  With `cache` (a JSON file path)
-    a pick stored for the same workload signature is used without trials,
-    and a new pick is stored.  Returns (best, trials): best = (waves, margin,
-    team), left set on bf; trials lists {"waves", "margin", "tile", "team",
-    "kernel_ms", "wall_ms"} (empty when the cache answered).
+    margins 1, 2, 3) x teams.  With trial_sweeps, waves and tile come from
+    fits of that many sweeps (with the first team size), then the team from
+    whole fits (max_iterations); without, every candidate is a whole fit.
+    With `cache` (a JSON file path) a pick stored for the same workload
+    signature is used without trials, and a new pick is stored.  Returns
+    (best, trials): best = (waves, margin, team), left set on bf; trials
+    lists {"waves", "margin", "tile", "team", "kernel_ms", "wall_ms",
+    "sweeps", "stage"} (empty when the cache answered).
     """
     key = workload_key(bf) if cache else None
     if cache:
@@ -100,25 +108,38 @@ def autotune(bf, prepare, waves=(2, 4, 8), margins=None, teams=(0,), tiles=None,
         margins = margins_for(tiles, bf.roi_size) if tiles is not None else (1, 2, 3)
     if not margins:
         raise ValueError("no tile can hold the ROI")
-    sweeps = max_iterations if trial_sweeps is None else trial_sweeps
     trials = []
     bf.setTiming(True)
-    for t in teams:
-        for w in waves:
-            for m in margins:
-                bf.setTuning(w, m, t)
-                prepare()
-                s0 = bf.stats()
-                t0 = time.perf_counter()
-                bf.doFit(sweeps)
-                wall = (time.perf_counter() - t0) * 1e3
-                ms = bf.stats()["lm_ms"] - s0["lm_ms"]
-                trials.append({"waves": w, "margin": m, "tile": bf.roi_size + 2 * m, "team": t, "kernel_ms": ms,
-                               "wall_ms": wall})
-                if log is not None:
-                    log("autotune: waves %d tile %d^2 team %d: fit kernel %.1f ms (%d sweeps)" %
-                        (w, bf.roi_size + 2 * m, t, ms, sweeps))
-    b = min(trials, key=lambda r: r["kernel_ms"])
+
+    def trial(w, m, t, sweeps, stage):
+        bf.setTuning(w, m, t)
+        prepare()
+        s0 = bf.stats()
+        t0 = time.perf_counter()
+        bf.doFit(sweeps)
+        wall = (time.perf_counter() - t0) * 1e3
+        ms = bf.stats()["lm_ms"] - s0["lm_ms"]
+        trials.append({"waves": w, "margin": m, "tile": bf.roi_size + 2 * m, "team": t, "kernel_ms": ms,
+                       "wall_ms": wall, "sweeps": sweeps, "stage": stage})
+        if log is not None:
+            log("autotune: waves %d tile %d^2 team %d: fit kernel %.1f ms (%d sweeps)" %
+                (w, bf.roi_size + 2 * m, t, ms, sweeps))
+        return ms
+
+    if trial_sweeps is None:
+        for t in teams:
+            for w in waves:
+                for m in margins:
+                    trial(w, m, t, max_iterations, 0)
+        final = trials
+    else:
+        # stage 1: waves and tile on short fits with the first team size
+        _, w1, m1 = min((trial(w, m, teams[0], trial_sweeps, 1), w, m) for w in waves for m in margins)
+        # stage 2: the team size on whole fits
+        for t in teams:
+            trial(w1, m1, t, max_iterations, 2)
+        final = [r for r in trials if r["stage"] == 2]
+    b = min(final, key=lambda r: r["kernel_ms"])
     best = (b["waves"], b["margin"], b["team"])
     bf.setTuning(*best)
     if cache:

@@ -48,11 +48,31 @@ def test_picks_the_fastest_and_sets_it():
     bf = FakeFitter()
     calls = []
     best, trials = dao_tune.autotune(bf, lambda: calls.append(1), waves=(2, 4, 8), tiles=(16, 18, 20, 22, 24),
-                                     teams=(0, 500), trial_sweeps=20)
+                                     teams=(0, 500))
     assert best == (4, 2, 0) and bf.tuning == best
     assert len(trials) == 3 * 4 * 2 and len(calls) == len(trials)
     assert set(t["tile"] for t in trials) == {18, 20, 22, 24}
-    assert all(n == 20 for n in bf.fits)
+    assert all(n == 2000 for n in bf.fits)
+
+
+class TailFitter(FakeFitter):
+    """Short fits favour team 0, whole fits (the long tail) team 500."""
+    def doFit(self, max_iterations=2000):
+        w, m, t = self.tuning
+        self.fits.append(max_iterations)
+        per_sweep = (1.0 if t == 0 else 1.6) if max_iterations <= 20 else (4.0 if t == 0 else 2.5)
+        self.ms += max_iterations * per_sweep + abs(w - 4) * 10.0 + abs(m - 2) * 7.0
+
+
+def test_two_stages_choose_the_team_on_whole_fits():
+    bf = TailFitter()
+    best, trials = dao_tune.autotune(bf, lambda: None, waves=(2, 4, 8), tiles=(18, 20, 22), teams=(0, 500),
+                                     trial_sweeps=10)
+    assert best == (4, 2, 500) and bf.tuning == best
+    stage1 = [t for t in trials if t["stage"] == 1]
+    stage2 = [t for t in trials if t["stage"] == 2]
+    assert len(stage1) == 9 and all(t["sweeps"] == 10 and t["team"] == 0 for t in stage1)
+    assert len(stage2) == 2 and all(t["sweeps"] == 2000 and (t["waves"], t["margin"]) == (4, 2) for t in stage2)
 
 
 def test_no_tile_holds_the_roi():
