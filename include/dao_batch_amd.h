@@ -91,6 +91,10 @@ int daoBatchSetCalibration(daoBatch *b, int frame, const double *rqe, const doub
 /* mFitNewImage / mFitNewBackground (multi_fit.c:1360-1408) for every frame. */
 int daoBatchNewImages(daoBatch *b, const double *images, int on_device);
 int daoBatchNewBackgrounds(daoBatch *b, const double *backgrounds, int on_device);
+/* daoBatchNewImages and daoBatchNewBackgrounds in one pass over the pixels
+ * (the same values; no counterpart in the reference, which sets image and
+ * background in two calls). */
+int daoBatchNewFrames(daoBatch *b, const double *images, const double *backgrounds, int on_device);
 /* mFitAnscombeTransformImage for every frame (ALS models). */
 int daoBatchAnscombe(daoBatch *b);
 

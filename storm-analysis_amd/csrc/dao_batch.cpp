@@ -180,6 +180,23 @@ int daoBatchNewBackgrounds(daoBatch* b, const double* bgs, int on_device) {
   });
 }
 
+int daoBatchNewFrames(daoBatch* b, const double* images, const double* backgrounds, int on_device) {
+  return dao::guarded([&] {
+    Engine& e = *b->eng;
+    const double *di = images, *db = backgrounds;
+    if (!on_device) {
+      const long long n = e.frame_pixels_total();
+      double* t = tmp(b, 2 * sizeof(double) * n);
+      DAO_CHECK(hipMemcpyAsync(t, images, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+      DAO_CHECK(hipMemcpyAsync(t + n, backgrounds, sizeof(double) * n, hipMemcpyHostToDevice, e.st));
+      di = t;
+      db = t + n;
+    }
+    e.new_frames_all_dev(di, db);
+    e.sync();
+  });
+}
+
 int daoBatchAnscombe(daoBatch* b) {
   return dao::guarded([&] {
     Engine& e = *b->eng;

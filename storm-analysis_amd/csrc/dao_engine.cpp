@@ -1001,6 +1001,21 @@ void Engine::new_images_all_dev(const double* img) {
   upload_frames();
 }
 
+void Engine::new_frames_all_dev(const double* img, const double* bg) {
+  if (packed()) {
+    launch_new_frame_pixels(px, 0, npix, img, bg, st);
+  } else {
+    long long at = 0;
+    for (int f = 0; f < F; f++) {
+      long long n = (long long)fr[f].W * fr[f].H;
+      launch_new_frame_pixels(px, fr[f].pix0, n, img + at, bg + at, st);
+      at += n;
+    }
+  }
+  for (int f = 0; f < F; f++) fr[f].nfit = 0;
+  upload_frames();
+}
+
 void Engine::new_backgrounds_all_dev(const double* bg) {
   if (packed()) {
     launch_new_background(px, 0, npix, bg, st);

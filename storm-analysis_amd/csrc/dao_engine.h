@@ -79,6 +79,8 @@ void launch_commit(const Sched& sc, const Peaks& pk, int* frame_counters, hipStr
 void launch_collect_escapes(const Sched& sc, const Peaks& pk, int4* override_box, hipStream_t st);
 void launch_new_image(const Pix& px, long long pix0, long long n, const double* img, hipStream_t st);
 void launch_new_background(const Pix& px, long long pix0, long long n, const double* bg, hipStream_t st);
+void launch_new_frame_pixels(const Pix& px, long long pix0, long long n, const double* img, const double* bg,
+                             hipStream_t st);
 void launch_set_calibration(const Pix& px, long long pix0, long long n, const double* rqe, const double* scm,
                             hipStream_t st);
 void launch_set_calibration_const(const Pix& px, long long pix0, long long n, double rqe, double scm,
@@ -237,6 +239,8 @@ class Engine {
   // every frame at once from device arrays concatenated at the frames'
   // current sizes
   void new_images_all_dev(const double* img_dev);
+  // new_images_all_dev and new_backgrounds_all_dev in one pass
+  void new_frames_all_dev(const double* img_dev, const double* bg_dev);
   void new_backgrounds_all_dev(const double* bg_dev);
   long long frame_pixels_total() const;
   // serving: re-size slot f within its capacity (false if it does not fit),

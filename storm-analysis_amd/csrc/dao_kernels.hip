@@ -233,6 +233,25 @@ __global__ void k_new_background(Pix px, long long pix0, long long n, const doub
   px.rw[k].stale = 1;
 }
 
+// mFitNewImage and mFitNewBackground together (k_new_image, then
+// k_new_background, in one pass over the pixel records).
+__global__ void k_new_frame_pixels(Pix px, long long pix0, long long n, const double* img, const double* bg) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long k = pix0 + i;
+  PixRO r = px.ro[k];
+  r.x = img[i] * r.rqe + r.scm;
+  r.bgest = bg[i] * r.rqe;
+  px.ro[k] = r;
+  PixRW w;
+  w.f = 0.0;
+  w.bg = 0.0;
+  w.tfi = 0.0;
+  w.cnt = 0;
+  w.stale = 1;
+  px.rw[k] = w;
+}
+
 // Calibration of pixels [pix0, pix0 + n) (mFitInitialize, multi_fit.c:971-981);
 // the next image is scaled with it.
 __global__ void k_set_calibration(Pix px, long long pix0, long long n, const double* rqe, const double* scm) {
@@ -676,6 +695,12 @@ void launch_collect_escapes(const Sched& sc, const Peaks& pk, int4* override_box
 void launch_new_image(const Pix& px, long long pix0, long long n, const double* img, hipStream_t st) {
   k_new_image<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, img);
 }
+void launch_new_frame_pixels(const Pix& px, long long pix0, long long n, const double* img, const double* bg,
+                             hipStream_t st) {
+  if (n <= 0) return;
+  k_new_frame_pixels<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, img, bg);
+}
+
 void launch_new_background(const Pix& px, long long pix0, long long n, const double* bg, hipStream_t st) {
   k_new_background<<<flat_grid(n), 256, 0, st>>>(px, pix0, n, bg);
 }

@@ -52,7 +52,7 @@ class FrameFitError(MultiFitterException):
 
 _FRAME_FAILED = -5
 _BATCH_ENTRY_POINTS = (
-    "daoBatchCreate", "daoBatchSetCalibration", "daoBatchNewImages", "daoBatchNewBackgrounds",
+    "daoBatchCreate", "daoBatchSetCalibration", "daoBatchNewImages", "daoBatchNewBackgrounds", "daoBatchNewFrames",
     "daoBatchAnscombe", "daoBatchNewPeaks", "daoBatchFit", "daoBatchGetPeaks", "daoBatchGetAllPeaks",
     "daoBatchGetAllPeaksDevice", "daoBatchGetFitImage", "daoBatchGetFitImages", "daoBatchRemoveErrorPeaks",
     "daoBatchLoadFrame", "daoBatchClearFrame", "daoBatchLoadFrames", "daoBatchClearFrames",
@@ -87,6 +87,7 @@ def loadBatchC():
         lib.daoBatchSetCalibration.argtypes = [vp, ctypes.c_int, vp, vp]
         lib.daoBatchNewImages.argtypes = [vp, vp, ctypes.c_int]
         lib.daoBatchNewBackgrounds.argtypes = [vp, vp, ctypes.c_int]
+        lib.daoBatchNewFrames.argtypes = [vp, vp, vp, ctypes.c_int]
         lib.daoBatchAnscombe.argtypes = [vp]
         lib.daoBatchNewPeaks.argtypes = [vp, vp, i32p, ctypes.c_char_p, ctypes.c_int]
         lib.daoBatchFit.argtypes = [vp, ctypes.c_int]
@@ -229,6 +230,16 @@ class BatchFitter(object):
     def newBackgrounds(self, backgrounds):
         p, dev, keep = _ptr(self._concat(backgrounds), numpy.float64)
         self.clib.daoBatchNewBackgrounds(self.handle, p, dev)
+
+    def newFrames(self, images, backgrounds):
+        """newImages and newBackgrounds in one pass over the pixels (same values)."""
+        pi, di, ki = _ptr(self._concat(images), numpy.float64)
+        pb, db, kb = _ptr(self._concat(backgrounds), numpy.float64)
+        if di != db:
+            raise MultiFitterException("images and backgrounds must both be host or both be device arrays")
+        self.clib.daoBatchNewFrames(self.handle, pi, pb, di)
+        if self.error_model == "ALS":
+            self.clib.daoBatchAnscombe(self.handle)
 
     def newPeaks(self, peaks, peaks_type, counts=None):
         """
@@ -406,12 +417,23 @@ class BatchFitter(object):
         d["status"] = status
         return d
 
-    def getAllPeaks(self):
-        """(rows n x 9 in PEAK_COLUMNS order, status, per-frame counts), frames in order."""
+    def getAllPeaks(self, out=None, status=None):
+        """
+        (rows n x 9 in PEAK_COLUMNS order, status, per-frame counts), frames in
+        order.  out / status: optional host arrays of at least n rows to fill
+        (for example in pinned memory, which the device writes directly); the
+        returned arrays are their first n rows.
+        """
         counts = numpy.array([self.getNFit(f) for f in range(self.n_frames)], dtype=numpy.int64)
         n = int(counts.sum())
-        out = numpy.zeros((n, 9))
-        status = numpy.zeros(n, dtype=numpy.int32)
+        if out is None or status is None:
+            out = numpy.zeros((n, 9))
+            status = numpy.zeros(n, dtype=numpy.int32)
+        else:
+            if out.shape[0] < n or out.shape[1:] != (9,) or out.dtype != numpy.float64 or status.shape[0] < n or \
+                    status.dtype != numpy.int32 or not out.flags.c_contiguous or not status.flags.c_contiguous:
+                raise MultiFitterException("getAllPeaks: out must be n x 9 float64 and status n int32, contiguous")
+            out, status = out[:n], status[:n]
         if n:
             self.clib.daoBatchGetAllPeaks(self.handle, out, status)
         return out, status, counts

@@ -198,3 +198,51 @@ def test_tuning_keeps_results_and_autotune_picks_a_candidate():
     print("autotune trials:", trials)
     assert best in [(t["waves"], t["margin"], t["team"]) for t in trials] and len(trials) == 8
     bf.cleanup()
+
+
+@pytest.mark.gpu
+def test_new_frames_in_one_pass_equal_image_then_background():
+    """daoBatchNewFrames (image and background in one pass) gives the fit of newImages + newBackgrounds,
+    from host and from device arrays; getAllPeaks fills caller buffers (pinned host memory) as it fills
+    its own arrays."""
+    import torch
+    frames = scmos_frames()
+    kw = {"roi_size": 18, "sigma_range": [0.8, 6.0]}
+    bgs = [numpy.full(fr.image.shape, 100.0) for fr in frames]
+    shapes = [fr.image.shape for fr in frames]
+    caps = [max(1, len(fr.peaks["x"])) for fr in frames]
+
+    def fit(load):
+        bf = dao_batch.BatchFitter(shapes, caps, model="3d", **kw)
+        for k, fr in enumerate(frames):
+            bf.setCalibration(k, rqe=fr.rqe, scmos_cal=fr.scmos_cal)
+        load(bf)
+        bf.newPeaks([fr.peaks for fr in frames], "hdf5")
+        bf.doFit()
+        own = bf.getAllPeaks()
+        n = own[0].shape[0]
+        rows = torch.empty((n + 5, 9), dtype=torch.float64, pin_memory=True).numpy()
+        status = torch.empty(n + 5, dtype=torch.int32, pin_memory=True).numpy()
+        given = bf.getAllPeaks(rows, status)
+        bf.cleanup()
+        numpy.testing.assert_array_equal(own[0], given[0])
+        numpy.testing.assert_array_equal(own[1], given[1])
+        return own
+
+    def two_calls(bf):
+        bf.newImages([fr.image for fr in frames])
+        bf.newBackgrounds(bgs)
+
+    def one_call_host(bf):
+        bf.newFrames([fr.image for fr in frames], bgs)
+
+    def one_call_device(bf):
+        img = torch.from_numpy(numpy.concatenate([fr.image.ravel() for fr in frames])).cuda()
+        bg = torch.from_numpy(numpy.concatenate([b.ravel() for b in bgs])).cuda()
+        bf.newFrames(img, bg)
+
+    ref = fit(two_calls)
+    for load in (one_call_host, one_call_device):
+        got = fit(load)
+        numpy.testing.assert_array_equal(got[0], ref[0])
+        numpy.testing.assert_array_equal(got[1], ref[1])
