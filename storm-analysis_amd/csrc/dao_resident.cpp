@@ -152,6 +152,7 @@ daoResident* daoResidentCreate(daoBatch* b, int workgroups, int max_iterations, 
       dao::fail(dao::kErrArg, "bad resident fitter arguments");
     Engine& e = *b->eng;
     if (e.m.errm != dao::EM_MLE) dao::fail(dao::kErrArg, "the resident fitter supports the MLE error model only");
+    if (e.F >= (1 << dao::kResMemberShift)) dao::fail(dao::kErrArg, "too many slots for the ticket encoding");
     r = new daoResident();
     try {
       r->b = b;
@@ -303,9 +304,15 @@ int daoResidentWait(daoResident* r, double timeout_seconds, int max_slots, int32
         if (found > 0) return;
         // the workgroups left (idle) while requests wait: start them again
         if (waiting && (spin & 255) == 0) r->ensure_running();
+        if (!waiting) spin = -1;  // nothing outstanding: sleep below
       }
       if (clock::now() >= t_end) return;
-      if (spin > 64) std::this_thread::yield();
+      if (spin < 0) {
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+        spin = 0;
+      } else if (spin > 64) {
+        std::this_thread::yield();
+      }
     }
   });
   return rc != dao::kOk ? rc : found;
