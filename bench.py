@@ -51,10 +51,10 @@ ALG_BYTES_PER_PIXEL = 36
 # --copy-calibration 1024`: FETCH_SIZE reads half the bytes moved, WRITE_SIZE
 # all of them).
 MEASURED_TRAFFIC = {
-    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "5a3e381",
-              "fetch_kib": 101096004.875, "write_kib": 102444974.34375, "fetch_scale": 2.0, "write_scale": 1.0},
-    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "5a3e381",
-             "fetch_kib": 115199337.4375, "write_kib": 115048283.0, "fetch_scale": 2.0, "write_scale": 1.0},
+    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "6549cf7",
+              "fetch_kib": 94116463.6875, "write_kib": 103177921.5625, "fetch_scale": 2.0, "write_scale": 1.0},
+    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "6549cf7",
+             "fetch_kib": 93883046.0, "write_kib": 103196481.75, "fetch_scale": 2.0, "write_scale": 1.0},
 }
 # accuracy bar of mixed precision against FP64 for x and y
 TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
