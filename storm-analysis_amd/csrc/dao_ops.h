@@ -474,6 +474,32 @@ __device__ inline void stamp_pixel(double& f, double& bg, double h, double psf, 
 // scmos at [(2PPL+s)*64+l], each lane touching only its own entries -- then
 // the shape table (64 (offset, exp) pairs, written by lane_shape) and the
 // reduced J and packed H of the turn (uniform values).
+// The lane's ROI geometry in a frame: slot s holds ROI pixel l + 64 s of the
+// ROI table (column | row << 8 in rxy, row * W + column in koff); bit s of
+// okm is set when that pixel exists.  The same for every turn of a frame, so
+// the frame fitter reads the table once per run instead of once per turn.
+template <int PPL>
+struct RoiGeo {
+  int rxy[PPL];
+  int koff[PPL];
+  unsigned okm;
+
+  __device__ void make(const Model& m, const int* roi_tab, const Frame& fr) {
+    const int l = lane_id();
+    okm = 0;
+#pragma unroll
+    for (int s = 0; s < PPL; s++) {
+      const int j = l + s * kWave;
+      const bool o = j < m.nroi;
+      const int v = o ? *gptr(roi_tab + j) : 0;
+      const int cx = v & 0xffff, cy = v >> 16;
+      rxy[s] = cx | (cy << 8);
+      koff[s] = cy * fr.W + cx;
+      if (o) okm |= 1u << s;
+    }
+  }
+};
+
 template <int PPL>
 struct RoiLds {
   static constexpr int kTab = 3 * PPL * kWave;
@@ -503,21 +529,20 @@ struct Roi {
   __device__ double& rqe(int s) { return lds[(PPL + s) * kWave + lane_id()]; }
   __device__ double& scm(int s) { return lds[(2 * PPL + s) * kWave + lane_id()]; }
 
-  __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
-    int l = lane_id();
-    okm = 0;
+  __device__ void init(const RoiGeo<PPL>& g, double* slice) {
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      int j = l + s * kWave;
-      bool o = j < m.nroi;
-      int v = o ? *gptr(roi_tab + j) : 0;
-      int cx = v & 0xffff, cy = v >> 16;
-      rxy[s] = cx | (cy << 8);
-      koff[s] = cy * fr.W + cx;
-      if (o) okm |= 1u << s;
+      rxy[s] = g.rxy[s];
+      koff[s] = g.koff[s];
     }
+    okm = g.okm;
     lds = slice;
     valid = false;
+  }
+  __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
+    RoiGeo<PPL> g;
+    g.make(m, roi_tab, fr);
+    init(g, slice);
   }
 
   // Loads the ROI at (xi, yi) (pixels through M); with snap, also journals
@@ -920,22 +945,21 @@ struct RoiM {
   __device__ double* red() { return lds + RoiLdsM<PPL>::kRed; }
   __device__ double& d0(int s) { return lds[RoiLdsM<PPL>::kD0 + s * kWave + lane_id()]; }
 
-  __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
-    int l = lane_id();
-    okm = 0;
-    d0ok = false;
+  __device__ void init(const RoiGeo<PPL>& g, double* slice) {
 #pragma unroll
     for (int s = 0; s < PPL; s++) {
-      int j = l + s * kWave;
-      bool o = j < m.nroi;
-      int v = o ? *gptr(roi_tab + j) : 0;
-      int cx = v & 0xffff, cy = v >> 16;
-      rxy[s] = cx | (cy << 8);
-      koff[s] = cy * fr.W + cx;
-      if (o) okm |= 1u << s;
+      rxy[s] = g.rxy[s];
+      koff[s] = g.koff[s];
     }
+    okm = g.okm;
+    d0ok = false;
     lds = slice;
     valid = false;
+  }
+  __device__ void init(const Model& m, const int* roi_tab, const Frame& fr, double* slice) {
+    RoiGeo<PPL> g;
+    g.make(m, roi_tab, fr);
+    init(g, slice);
   }
 
   template <class M = MemL1, class JM = M>
@@ -1219,26 +1243,35 @@ __device__ constexpr int upk(int r, int c) {  // packed index of (r, c), r <= c
 // storage = Cholesky of the symmetric matrix given by the upper triangle,
 // whose diagonal is scaled by (1 + lambda) (multi_fit.c:1174-1185).
 // Non-zero on a pivot that is not > 0 (or NaN) or a NaN in the solution.
+// Branch-free: H (in LDS) is read into registers at once, and a failed pivot
+// only sets the result (the columns after it compute values nobody uses);
+// per-column early exits made every LDS read wait for the branch before it.
 template <int N>
 __device__ inline int chol_solve(const double* H, double lam, double* b) {
+  constexpr int NH = N * (N + 1) / 2;
+  double h[NH];
+#pragma unroll
+  for (int q = 0; q < NH; q++) h[q] = H[q];
   double L[N][N];
   double inv[N];  // 1 / L[j][j]
+  bool ok = true;
 #pragma unroll
   for (int j = 0; j < N; j++) {
-    double s = (1.0 + lam) * H[upk<N>(j, j)];
+    double s = (1.0 + lam) * h[upk<N>(j, j)];
 #pragma unroll
     for (int k = 0; k < j; k++) s -= L[j][k] * L[j][k];
-    if (!(s > 0.0)) return j + 1;
+    ok = ok && (s > 0.0);
     const double d = sqrt(s);
     inv[j] = 1.0 / d;
 #pragma unroll
     for (int i = j + 1; i < N; i++) {
-      double t = H[upk<N>(j, i)];
+      double t = h[upk<N>(j, i)];
 #pragma unroll
       for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
       L[i][j] = t * inv[j];
     }
   }
+  if (!ok) return 1;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     double t = b[i];
@@ -1367,7 +1400,8 @@ __device__ inline void reset_from(PeakState& w, const PeakRec* r) {
 // cannot be undone or leave its box (its box is the whole frame) needs none.
 template <int PPL, int SHAPE, class R = Roi<PPL>, class M = MemL1, int EM = -1>
 __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Frame& fr, const int* roi_tab,
-                       double* slice, int p, int4 box, int* d, double* jr, int& nsnap, double* stage = nullptr) {
+                       double* slice, int p, int4 box, int* d, double* jr, int& nsnap, double* stage = nullptr,
+                       const RoiGeo<PPL>* geo = nullptr) {
   constexpr int N = SHAPE == SH_2DFIXED ? 4 : (SHAPE == SH_3D ? 6 : 5);
   PeakRec* rec = pk.rec + p;
   nsnap = 0;
@@ -1382,7 +1416,10 @@ __device__ int lm_turn(const Model& m, const Pix& px, const Peaks& pk, const Fra
   }
 
   R c;
-  c.init(m, roi_tab, fr, slice);
+  if (geo)
+    c.init(*geo, slice);
+  else
+    c.init(m, roi_tab, fr, slice);
   if constexpr (std::is_same<M, MemL1>::value) {
     c.template load<M>(px, fr, w.xi, w.yi, (jr && stage) ? stage : jr);
   } else if (jr && stage) {
@@ -1893,6 +1930,8 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int n, const i
   using R = typename std::conditional<MP, RoiM<PPL>, Roi<PPL>>::type;
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   for (int k = threadIdx.x; k < n; k += blockDim.x) a.op_done[fr.peak0 + k] = 0;
+  RoiGeo<PPL> geo;
+  if constexpr (kind == OP_LM) geo.make(a.m, a.roi_tab, fr);
   int from = 0;
   for (;;) {
     if (threadIdx.x < (blockDim.x >> 6)) sh.cur[threadIdx.x] = kIdle;
@@ -1927,7 +1966,7 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int n, const i
       double* const jr = kind == OP_LM ? op_journal<PPL>(a, fr, slot, b) : nullptr;
       double* const stage = fit_frames_stage<PPL>() > 0 ? slice + fit_frames_slice<PPL, MP>() : nullptr;
       if constexpr (kind == OP_LM) {
-        r = lm_turn<PPL, SHAPE, R, MemL1, EM>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage);
+        r = lm_turn<PPL, SHAPE, R, MemL1, EM>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, b, d, jr, nsnap, stage, &geo);
       } else if constexpr (kind == OP_RECENTER) {
         recenter_turn<PPL, R>(a.m, a.px, a.pk, fr, a.roi_tab, slice, p, d);
       } else if constexpr (kind == OP_NEWPEAK) {
