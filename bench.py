@@ -98,7 +98,8 @@ def parse():
     p.add_argument("--serve-setups", default="resident,latency",
                    help="serving set-ups to run: resident (frames admitted into a running kernel), "
                         "throughput, latency (a launch per batch)")
-    p.add_argument("--resident-slots", type=int, default=64, help="frame slots of the resident server")
+    p.add_argument("--resident-slots", type=int, default=0,
+                   help="frame slots of the resident server (0: one per resident workgroup)")
     p.add_argument("--tune-sweeps", type=int, default=10,
                    help="sweeps of each autotuner trial on the uneven batch (its first sweeps hold most of the work)")
     p.add_argument("--resident-team", type=int, default=64,
@@ -785,8 +786,9 @@ def serve_leg(args, precision, log):
     from storm_analysis_amd.sa_library import dao_serve
     from storm_analysis_amd.diagnostics import serve_sim
     pools = serve_sim.make_pools((256, 512, 1024))
-    setups = {"resident": dict(slots=args.resident_slots, margin=2),
-              "resident_team": dict(slots=args.resident_slots, margin=1, team=args.resident_team),
+    slots = args.resident_slots or None  # None: one per resident workgroup
+    setups = {"resident": dict(slots=slots, margin=2),
+              "resident_team": dict(slots=slots, margin=1, team=args.resident_team),
               "throughput": dict(workers=8, slots=32, team=0, margin=2),
               "latency": dict(workers=args.serve_workers, slots=args.serve_slots, team=args.serve_team, margin=1)}
     setups = {k: v for k, v in setups.items() if k in args.serve_setups.split(",")}

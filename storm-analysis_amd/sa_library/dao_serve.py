@@ -316,11 +316,18 @@ def loadResidentC():
     return _res_lib
 
 
+def resident_workgroups():
+    """Workgroups of a resident kernel by default: one per compute unit but 8."""
+    import torch
+    return max(1, torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count - 8)
+
+
 class ResidentServer(object):
     """
     Continuous batching inside one running kernel.  `slots` frame slots of
     max_shape (height, width) and max_peaks peaks (fewer if hbm_budget_gb
-    holds fewer) on one device; a resident kernel (include/dao_resident_amd.h)
+    holds fewer) on one device -- by default one per resident workgroup, so
+    that no request waits for a slot while a workgroup is free; a resident kernel (include/dao_resident_amd.h)
     whose workgroups take each request as soon as one is free, fit it
     (new image, background, start peaks, doFit) and write its peaks to
     pinned host memory.  A request waits in a host queue only while every
@@ -331,12 +338,14 @@ class ResidentServer(object):
     (BatchFitter) bit for bit.  MLE only; the remaining keywords are
     BatchFitter's (model, precision, roi_size, wx_params, ...).
     """
-    def __init__(self, slots=64, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None, max_iterations=2000,
+    def __init__(self, slots=None, max_shape=(1024, 1024), max_peaks=4000, hbm_budget_gb=None, max_iterations=2000,
                  margin=-1, workgroups=0, idle_seconds=0.5, wait_limit=0, team=0, **fitter_kw):
         self.clib = loadResidentC()
         self.max_shape = tuple(max_shape)
         self.max_peaks = max_peaks
         roi_size = fitter_kw.get("roi_size", 18)
+        if slots is None:
+            slots = workgroups if workgroups > 0 else resident_workgroups()
         _, slots = pool_size(1, slots, max_shape, max_peaks, hbm_budget_gb, roi_size)
         self.slots = slots
         self.workers = 1
