@@ -1951,14 +1951,15 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int n, const i
         break;
       }
       const int slot = fr.peak0 + j;
+      // box and peak index in one round trip
       const int4 b = uniform_i4(a.box[slot]);
+      const int p = uniform_i(list[slot]);
       if (lane == 0) sh.curbox[wv] = b;
       fpw_set_cur(&sh.cur[wv], j);
       if (!fpw_wait<NW>(a, f, j, b, sh)) {
         fpw_set_cur(&sh.cur[wv], kIdle);
         break;
       }
-      const int p = uniform_i(list[slot]);
       int d[C_COUNT];
 #pragma unroll
       for (int q = 0; q < C_COUNT; q++) d[q] = 0;
