@@ -141,7 +141,7 @@ struct OpArgs {
 struct TeamCtl {
   int phase;   // generation: raised by each transition (a new run, a rollback, the end)
   int lead;    // the generation whose transition a workgroup has claimed
-  int kind;    // OP_LM, OP_RECENTER or kTeamDone
+  int kind;    // OP_LM, OP_RECENTER, kTeamDone or kTeamHandoff
   int n;       // ops of the current run
   int next;    // next op to hand out
   int done;    // ops of the run finished
@@ -152,6 +152,10 @@ struct TeamCtl {
   int pad[6];
 };
 constexpr int kTeamDone = -1;
+// The team stopped at an LM run of at most FitArgs::team_solo ops (run_list,
+// nrun, it and sweeps as the transition left them): k_fit_frames with
+// FitArgs::team_resume finishes the frame with one workgroup.
+constexpr int kTeamHandoff = -2;
 constexpr int kTeamSeats = 8;  // seats (one per wave) reserved per workgroup
 constexpr int kMaxTeam = 64;   // workgroups per frame
 
@@ -206,6 +210,8 @@ struct FitArgs {
   const int* team_block_frame;  // [workgroups]: frame of the workgroup
   const int* team_block_seat;   // [workgroups]: first seat of the workgroup
   const int* team_seat0;        // [F + 1]: frame f's seats are [team_seat0[f], team_seat0[f + 1])
+  int team_solo = 0;            // hand an LM run of at most this many ops to one workgroup (0: never)
+  int team_resume = 0;          // k_fit_frames: continue the frames a team handed off (kTeamHandoff)
 };
 
 // One frame's share of a k_add_peaks launch: n start rows (device memory,

@@ -586,7 +586,17 @@ int Engine::fit_frames(int max_more) {
   }
   if (timing) DAO_CHECK(hipEventRecord(ev0, st));
   if (nblocks > 0) {
+    // A team hands a frame's tail (an LM run of at most team_solo ops) to
+    // one workgroup: the resume launch, queued behind the team's, finishes
+    // those frames and leaves at once for the others.
+    a.team_solo = team_solo;
     launch_fit_team(a, nblocks, st);
+    if (team_solo > 0) {
+      DAO_CHECK(hipGetLastError());
+      FitArgs b = a;
+      b.team_resume = 1;
+      launch_fit_frames(b, (int)list.size(), st);
+    }
   } else {
     launch_fit_frames(a, (int)list.size(), st);
   }

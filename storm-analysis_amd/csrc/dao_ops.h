@@ -2041,18 +2041,25 @@ __device__ void fpw_run(const FitArgs& a, int f, const Frame& fr, int n, const i
 // (a.max_more = 0: exactly one sweep).  Counter increments accumulate in
 // sh.cnt; sh.failed is set if a scheduler wait gave up.  Returns the sweeps
 // run.
+//
+// With n0 >= 0 the fit resumes where a team handed it off: iteration it0,
+// whose list of n0 RUNNING peaks is already in run_list (built and checked
+// against max_more as the loop below would).
 template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
-__device__ int fit_frame(const FitArgs& a, int f, const Frame& fr, FpwShared<NW>& sh, double* slice) {
+__device__ int fit_frame(const FitArgs& a, int f, const Frame& fr, FpwShared<NW>& sh, double* slice, int it0 = 0,
+                         int n0 = -1) {
   const Model& m = a.m;
   int sweeps = 0;
-  int n = 0;  // RUNNING peaks at the sweep's start: a.run_list[peak0 .. peak0 + n)
-  for (int it = 0;; it++) {
+  int n = n0 > 0 ? n0 : 0;  // RUNNING peaks at the sweep's start: a.run_list[peak0 .. peak0 + n)
+  for (int it = it0;; it++) {
     // a wait gave up (every wave passed a barrier since): the frame's state
     // is not the sequential one, so it stops and the host reports it
     if (sh.failed) break;
-    n = it == 0 ? fpw_list(a, fr, SEL_RUNNING, sh.wsum, nullptr, 0, a.run_list)
-                : fpw_list(a, fr, SEL_RUNNING, sh.wsum, a.run_list, n, a.run_list);
-    if (it > 0 && (n == 0 || it > a.max_more)) break;
+    if (n0 < 0 || it > it0) {
+      n = it == 0 ? fpw_list(a, fr, SEL_RUNNING, sh.wsum, nullptr, 0, a.run_list)
+                  : fpw_list(a, fr, SEL_RUNNING, sh.wsum, a.run_list, n, a.run_list);
+      if (it > 0 && (n == 0 || it > a.max_more)) break;
+    }
     if (n == 0) continue;  // the unconditional first sweep found nothing
     sweeps++;
     // reserved boxes (ROI at the sweep's start + margin) and backups
@@ -2093,7 +2100,10 @@ __device__ int fit_frame(const FitArgs& a, int f, const Frame& fr, FpwShared<NW>
   return sweeps;
 }
 
-template <int PPL, int SHAPE, int NW, bool MP, int EM = -1>
+// RESUME: the launch follows k_fit_team on the stream and continues only
+// the frames their team handed off (a separate instance, so that the
+// ordinary one compiles as if resuming did not exist).
+template <int PPL, int SHAPE, int NW, bool MP, int EM = -1, bool RESUME = false>
 __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   constexpr int kSlice = fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>();
   __shared__ FpwShared<NW> sh;
@@ -2101,13 +2111,21 @@ __global__ __launch_bounds__(NW * 64) void k_fit_frames(FitArgs a) {
   double* const slice = lds_dyn + (threadIdx.x >> 6) * kSlice;
   const int f = a.frame_list[blockIdx.x];
   const Frame fr = a.frames[f];
+  int it0 = 0, n0 = -1, sweeps0 = 0;
+  if constexpr (RESUME) {
+    const TeamCtl* ctl = a.team_ctl + f;
+    if (ctl->kind != kTeamHandoff) return;
+    it0 = ctl->it;
+    n0 = ctl->nrun;
+    sweeps0 = ctl->sweeps;
+  }
   if (threadIdx.x < C_COUNT) sh.cnt[threadIdx.x] = 0;
   if (threadIdx.x == 0) sh.failed = 0;
   __syncthreads();
-  const int sweeps = fit_frame<PPL, SHAPE, NW, MP, EM>(a, f, fr, sh, slice);
+  const int sweeps = fit_frame<PPL, SHAPE, NW, MP, EM>(a, f, fr, sh, slice, it0, n0);
   if (threadIdx.x < C_COUNT && sh.cnt[threadIdx.x] != 0)
     atomicAdd(&a.frame_counters[f * C_COUNT + threadIdx.x], sh.cnt[threadIdx.x]);
-  if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps;
+  if (threadIdx.x == 0) a.frame_sweeps[f] = sweeps0 + sweeps;
 }
 
 // The ROI box at origin (x0, y0), clipped to the frame.
@@ -2201,7 +2219,7 @@ struct TeamShared {
   int cnt[C_COUNT];
   int gen;   // generation of the current action
   int act;   // kActRun, kActLead or kActStop
-  int kind;  // the run's kind (OP_LM, OP_RECENTER, kTeamDone)
+  int kind;  // the run's kind (OP_LM, OP_RECENTER, kTeamDone, kTeamHandoff)
   int n;     // the run's op count
 };
 constexpr int kActRun = 0, kActLead = 1, kActStop = 2;
@@ -2480,6 +2498,14 @@ __device__ DAO_TEAM_COLD void team_transition(const FitArgs& a, const Frame& fr,
       if (nl == 0 || it > a.max_more) {
         nk = kTeamDone;
         nn = 0;
+      } else if (nl <= a.team_solo) {
+        // In a fit's long tail a few peaks stay RUNNING for hundreds of
+        // sweeps; through the team each such sweep pays two transitions
+        // (claim, quiet, lists, publish, every team mate's poll), about
+        // twice what one workgroup's sweep costs.  The team stops here and
+        // k_fit_frames resumes the frame from this run (team_resume).
+        nk = kTeamHandoff;
+        nn = 0;
       } else {
         team_boxes_lm(a, fr, nl);
         nk = OP_LM;
@@ -2542,7 +2568,7 @@ __device__ void team_fit(const FitArgs& a, int f, const Frame& fr, int s0, int s
       continue;  // g unchanged: the next await sees the new generation
     }
     g = gen;
-    if (kind == kTeamDone) break;
+    if (kind == kTeamDone || kind == kTeamHandoff) break;
     team_ops<PPL, SHAPE, MP, EM>(a, f, fr, ctl, g, kind, n, seat, sh, slice);
     __syncthreads();
   }
@@ -2595,18 +2621,25 @@ static inline dim3 flat_grid(long long n) { return dim3((unsigned)((n + 255) / 2
 
 // Launch k_fit_frames<PPL, SHP, NW, MP> for nframes frames with a.m.waves
 // waves per workgroup (0: NW; at most NW) and their LDS slices.
-template <int PPL, int SHP, int NW, bool MP, int EM = -1>
-void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
+template <int PPL, int SHP, int NW, bool MP, int EM, bool RESUME>
+void launch_fit_frames_k(const FitArgs& a, int nframes, hipStream_t st) {
   const int nw = a.m.waves > 0 && a.m.waves < NW ? a.m.waves : NW;
   const size_t bytes = sizeof(double) * (size_t)nw * (fit_frames_slice<PPL, MP>() + fit_frames_stage<PPL>());
   static size_t allowed = 64 * 1024;  // larger dynamic LDS must be enabled per kernel
   if (bytes > allowed) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP, EM>),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fit_frames<PPL, SHP, NW, MP, EM, RESUME>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess)
       return;  // the launch below would fail; the caller's error check reports it
     allowed = bytes;
   }
-  k_fit_frames<PPL, SHP, NW, MP, EM><<<nframes, nw * 64, bytes, st>>>(a);
+  k_fit_frames<PPL, SHP, NW, MP, EM, RESUME><<<nframes, nw * 64, bytes, st>>>(a);
+}
+template <int PPL, int SHP, int NW, bool MP, int EM = -1>
+void launch_fit_frames_t(const FitArgs& a, int nframes, hipStream_t st) {
+  if (a.team_resume)
+    launch_fit_frames_k<PPL, SHP, NW, MP, EM, true>(a, nframes, st);
+  else
+    launch_fit_frames_k<PPL, SHP, NW, MP, EM, false>(a, nframes, st);
 }
 
 // Launch k_add_peaks<PPL, NW> over njobs workgroups of nw waves.
