@@ -115,6 +115,11 @@ int daoBatchFailedFrames(daoBatch *b, int32_t *flags);
 /* Polls after which a scheduler wait of the fitter gives up and fails its
  * frame (0: the default, far beyond any wait of a correct schedule). */
 void daoBatchSetWaitLimit(daoBatch *b, long long polls);
+/* Teams (daoBatchSetTuning's team > 0) hand a frame's tail to one workgroup
+ * once an LM sweep would run at most `ops` peaks (a long tail of sweeps with
+ * few RUNNING peaks costs about twice as much through a team); 0: never,
+ * < 0: the default (32).  The fits do not depend on it. */
+void daoBatchSetTeamHandoff(daoBatch *b, int ops);
 
 /* Per-frame read-back.  out: nfit x 9 doubles per peak
  * {x, y, z, height, background, xsigma, ysigma, error, iterations}. */

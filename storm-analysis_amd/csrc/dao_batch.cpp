@@ -337,6 +337,8 @@ void daoBatchSetTuning(daoBatch* b, int waves, int margin, int team) {
 
 void daoBatchSetWaitLimit(daoBatch* b, long long polls) { b->eng->spin_limit = polls > 0 ? polls : 0; }
 
+void daoBatchSetTeamHandoff(daoBatch* b, int ops) { b->eng->team_solo = ops < 0 ? dao::kTeamSolo : ops; }
+
 int daoBatchClearFrame(daoBatch* b, int f) {
   return dao::guarded([&] {
     Engine& e = *b->eng;

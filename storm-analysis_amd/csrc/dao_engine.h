@@ -19,6 +19,9 @@
 
 namespace dao {
 
+// Default of Engine::team_solo (daoBatchSetTeamHandoff).
+constexpr int kTeamSolo = 32;
+
 // Error codes of the library's entry points (0: success).  No entry point
 // aborts the process: a failure inside the engine throws dao::Error, the
 // C-ABI function that was called catches it, records it as the calling
@@ -216,8 +219,10 @@ class Engine {
   long long spin_limit = 0;  // polls before a scheduler wait gives up (0: kSpinLimit)
   // teams hand a frame's LM runs of at most this many ops to one workgroup
   // (0: never): ~22 us per tail sweep against ~42 us through the team on
-  // 512^2 Z frames of 1000 emitters, measured on one MI355X
-  int team_solo = 8;
+  // 512^2 Z frames of 1000 emitters, measured on one MI355X; 32 and 64 gave
+  // the lowest single-frame latency there, and no change on a dense 2048^2
+  // frame or an uneven batch with teams of 500
+  int team_solo = kTeamSolo;
   bool use_rounds = false;  // multi-kernel executor instead of fit_frames
   // timing of the LM kernel (HIP events on the engine's stream)
   bool timing = false;

@@ -95,6 +95,7 @@ def loadBatchC():
         lib.daoBatchFailedFrames.argtypes = [vp, i32p]
         lib.daoBatchFailedFrames.restype = ctypes.c_int
         lib.daoBatchSetWaitLimit.argtypes = [vp, ctypes.c_longlong]
+        lib.daoBatchSetTeamHandoff.argtypes = [vp, ctypes.c_int]
         lib.daoBatchNFit.argtypes = [vp, ctypes.c_int]
         lib.daoBatchNFit.restype = ctypes.c_int
         lib.daoBatchGetPeaks.argtypes = [vp, ctypes.c_int, f64p, i32p]
@@ -357,6 +358,14 @@ class BatchFitter(object):
     def setWaitLimit(self, polls):
         """Polls after which a scheduler wait gives up and fails its frame (0: default)."""
         self.clib.daoBatchSetWaitLimit(self.handle, int(polls))
+
+    def setTeamHandoff(self, ops=-1):
+        """
+        With teams, hand a frame's tail to one workgroup once an LM sweep
+        would run at most `ops` peaks (0: never; -1: the default, 32).  The
+        fits do not depend on it.
+        """
+        self.clib.daoBatchSetTeamHandoff(self.handle, int(ops))
 
     def doFitFrames(self, active, max_iterations=2000):
         """

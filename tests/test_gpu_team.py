@@ -58,11 +58,12 @@ def test_team_3d_scmos_matches_oracle():
     assert_parity(g, r)
 
 
-def batch_run(frames, kw, bgs, waves=0, margin=-1, team=0, precision="fp64", model="3d"):
+def batch_run(frames, kw, bgs, waves=0, margin=-1, team=0, precision="fp64", model="3d", handoff=-1):
     shapes = [fr.image.shape for fr in frames]
     caps = [max(1, len(fr.peaks["x"])) for fr in frames]
     bf = dao_batch.BatchFitter(shapes, caps, model=model, precision=precision, **kw)
     bf.setTuning(waves, margin, team)
+    bf.setTeamHandoff(handoff)
     for k, fr in enumerate(frames):
         if fr.rqe is not None:
             bf.setCalibration(k, rqe=fr.rqe, scmos_cal=fr.scmos_cal)
@@ -88,6 +89,28 @@ def test_team_batch_uneven_scmos_matches_oracle(margin):
     for k, fr in enumerate(frames):
         r = oracle_fit(D.MultiFitter3D, kw, fr, bgs[k])
         assert out[k]["x"].size == r["x"].size
+        assert_frame_parity(out[k], r, counters[k])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("handoff", [0, 1, 10 ** 6])
+def test_team_handoff_keeps_fits(handoff):
+    """
+    A team hands a frame's tail -- its LM runs of at most `handoff` ops -- to
+    one workgroup (0: never; 10**6: from the first sweep on, so the resumed
+    fit starts at iteration 0).  FP64 fits are the default's to the bit and
+    the oracle's to the parity bar, counters included.
+    """
+    frames = scmos_frames()
+    kw = {"roi_size": 18, "sigma_range": [0.8, 6.0]}
+    bgs = [numpy.full(fr.image.shape, 100.0) for fr in frames]
+    out, counters = batch_run(frames, kw, bgs, margin=2, team=12, handoff=handoff)
+    ref, ref_c = batch_run(frames, kw, bgs, margin=2, team=12)
+    for k, fr in enumerate(frames):
+        for key in ref[k]:
+            numpy.testing.assert_array_equal(out[k][key], ref[k][key])
+        assert counters[k] == ref_c[k]
+        r = oracle_fit(D.MultiFitter3D, kw, fr, bgs[k])
         assert_frame_parity(out[k], r, counters[k])
 
 
