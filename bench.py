@@ -51,10 +51,10 @@ ALG_BYTES_PER_PIXEL = 36
 # --copy-calibration 1024`: FETCH_SIZE reads half the bytes moved, WRITE_SIZE
 # all of them).
 MEASURED_TRAFFIC = {
-    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "538d3a4",
-              "fetch_kib": 94256818.0, "write_kib": 105934633.8, "fetch_scale": 2.0, "write_scale": 1.0},
-    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "538d3a4",
-             "fetch_kib": 94057865.1, "write_kib": 105596814.2, "fetch_scale": 2.0, "write_scale": 1.0},
+    "mixed": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "79033c9",
+              "fetch_kib": 94242076.2, "write_kib": 105784091.6, "fetch_scale": 2.0, "write_scale": 1.0},
+    "fp64": {"frames": 500, "size": 512, "emitters": 1000, "margin": 2, "waves": 0, "commit": "79033c9",
+             "fetch_kib": 94054058.6, "write_kib": 105365188.1, "fetch_scale": 2.0, "write_scale": 1.0},
 }
 # accuracy bar of mixed precision against FP64 for x and y
 TOL_MIXED_XY_PX = 6.25e-3  # 1 nm at 160 nm pixels
