@@ -87,7 +87,7 @@ def parse():
                    help="seconds of simulated camera traffic per offered load for the serving leg (0: skip)")
     p.add_argument("--serve-loads", default="0.05,0.1,0.2",
                    help="offered loads of the serving leg, as fractions of 16 cameras at U(50, 400) frames/s")
-    p.add_argument("--latency-frames", type=int, default=30,
+    p.add_argument("--latency-frames", type=int, default=100,
                    help="frames fitted one at a time for the single-frame latency leg (0: skip)")
     p.add_argument("--team", type=int, default=64,
                    help="peaks per workgroup of the frame-per-team fitter in the latency and serving legs")
